@@ -1,0 +1,112 @@
+// Device-side data structures and kernel launchers of the Kriging likelihood
+// pipeline (gfx950).  Internal to liblkgpu.so: the public boundary is the C
+// ABI in include/lkgpu.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace lk {
+
+constexpr int NB = 64;     // tile / panel width (rows and columns of one block)
+constexpr int DMAX = 32;   // largest input dimension d supported by the kernels
+constexpr int PMAX = 33;   // largest trend width p (linear trend with d = 32)
+
+enum Kernel { K_GAUSS = 0, K_EXP = 1, K_MATERN32 = 2, K_MATERN52 = 3 };
+enum Noise { NOISE_NONE = 0, NOISE_NUGGET = 1 };
+
+// Status of one slot after an evaluation.
+enum SlotStatus {
+  ST_OK = 0,
+  ST_CHOL_FAILED = 1,   // a non-positive or NaN pivot
+  ST_RCOND_SUSPECT = 2, // exact rcond below threshold: the 1-norm estimator decides
+  ST_RCOND_LOW = 3,     // estimator confirmed rcond below threshold
+  ST_INACTIVE = 4,
+};
+
+// Static description of one batch slot: which design it evaluates and the
+// device memory it works in.  Rebuilt only when the slot -> problem mapping
+// changes.
+struct Slot {
+  const double* X;  // n x d, column-major (ld = n)
+  const double* y;  // n
+  const double* F;  // n x p, column-major (ld = n)
+  double* A;        // npad x npad: R, then its lower Cholesky factor L
+  double* W;        // npad x npad: L^-1 in the lower triangle; scratch above
+  double* V;        // npad x (p + 4): [F* | y* | r | E* | x]
+  double* red;      // reduction scratch (see RED_* offsets)
+  int n, npad, nblk, d, p;
+  int kernel, noise;
+  int pad_;
+};
+
+// Per-evaluation parameters of one slot (uploaded before each batch).
+struct SlotParams {
+  double theta[DMAX];
+  double alpha;      // nugget model: R = alpha*C + (1 - alpha)*I
+  double sigma2;     // fixed sigma2 (when !est_sigma2)
+  double nugget;     // fixed nugget (when !est_nugget)
+  double num_nugget; // jitter unit of the retry ladder
+  double min_rcond;
+  int inc;           // jitter steps already applied to the diagonal
+  int active;
+  int est_sigma2;
+  int est_nugget;
+  int rcond_check;
+  int want_grad;
+  int gamma_has_alpha; // gradient also over alpha
+  int pad_;
+};
+
+// Per-slot outputs (downloaded after each batch).
+struct SlotOut {
+  double ll;
+  double logdet;     // sum log L_ii
+  double sse;        // E*'E*
+  double sigma2;     // variance used to normalise the gradient
+  double norm1_L;
+  double norm1_Linv;
+  double rcond2;     // (1 / (|L|_1 |L^-1|_1))^2, or the estimator's value
+  double grad[DMAX + 1];
+  double beta[PMAX];
+  int status;
+  int pad_[3];
+};
+
+// Layout of Slot::red (doubles).
+constexpr int RED_LOGDET = 0;                 // nblk partial log-dets
+__host__ __device__ inline int red_sse_off(int nblk) { return nblk; }          // nblk partial SSE
+__host__ __device__ inline int red_colL_off(int nblk) { return 2 * nblk; }     // nblk partial |L|_1 column maxima
+__host__ __device__ inline int red_colW_off(int nblk) { return 3 * nblk; }     // nblk partial |L^-1|_1 column maxima
+__host__ __device__ inline int red_grad_off(int nblk) { return 4 * nblk; }     // tiles x GRAD_STRIDE adjoint partials
+constexpr int GRAD_STRIDE = 2 * DMAX + 4;
+__host__ __device__ inline size_t red_size(int nblk) {
+  return (size_t)4 * nblk + (size_t)nblk * (nblk + 1) / 2 * GRAD_STRIDE + 64;
+}
+
+struct Batch {
+  const Slot* slots;        // device
+  const SlotParams* params; // device
+  SlotOut* out;             // device
+  int nslots;
+  int nblk_max;             // grid extent (max nblk over slots)
+  int dmax;                 // max d over slots (selects the unrolled kernel variant)
+  hipStream_t stream;
+};
+
+// Launchers (each returns the hipGetLastError of the launch).
+hipError_t launch_build(const Batch& b);
+hipError_t launch_potrf_diag(const Batch& b, int k);
+hipError_t launch_potrf_panel(const Batch& b, int k);
+hipError_t launch_potrf_update(const Batch& b, int k);
+hipError_t launch_trtri_level(const Batch& b, int s, int step);
+hipError_t launch_solve_fy(const Batch& b);
+hipError_t launch_gls(const Batch& b);
+hipError_t launch_solve_e(const Batch& b);
+hipError_t launch_solve_x(const Batch& b);
+hipError_t launch_finalize_ll(const Batch& b);
+hipError_t launch_lauum_adjoint(const Batch& b);
+hipError_t launch_finalize_grad(const Batch& b);
+hipError_t launch_rcond_estimate(const Batch& b);
+
+int trtri_levels(int nblk);
+
+}  // namespace lk

@@ -1,0 +1,1160 @@
+// Kriging concentrated log-likelihood + adjoint gradient, FP64, gfx950.
+//
+// One "slot" is one LL(+grad) evaluation of one design at one parameter
+// vector.  Every kernel below works on a batch of slots (grid.y or grid.x =
+// slot) so that 64 multistart evaluations, or 256 uneven NestedKriging
+// submodels, run in the same launches.  Matrices live in HBM as FP64
+// column-major npad x npad (npad = n rounded up to 64, the padded part set to
+// the identity so that every kernel works on whole 64 x 64 tiles).
+//
+// Pipeline of one evaluation (reference: Kriging::_logLikelihood,
+// src/lib/Kriging.cpp:214-338, and KrigingImpl::populate_Model,
+// src/lib/KrigingImpl.cpp:73-125):
+//   build           R = factor * k(x_i - x_j; theta), unit (+jitter) diagonal
+//   potrf diag/panel/update   blocked right-looking Cholesky, FP64 MFMA
+//   trtri           L^-1 by recursive doubling, FP64 MFMA
+//   solve_fy, gls, solve_e, solve_x   F* = L^-1 F, y* = L^-1 y, GLS beta,
+//                   E* = L^-1 (y - F beta), x = L^-T E*, |L|_1, |L^-1|_1
+//   finalize_ll     log-det, SSE, LL, rcond check
+//   lauum_adjoint   R^-1 = L^-T L^-1 tile by tile on MFMA; each tile is
+//                   contracted at once with dR/dtheta_k recomputed from X
+//                   (all k, plus the nugget direction), never stored
+//   finalize_grad   deterministic reduction of the per-tile partial sums
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "lk_kernels.h"
+
+namespace lk {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int KC = 32;          // K-chunk staged in LDS per step
+constexpr int LDSS = NB + 4;    // LDS row stride (doubles) of a KC x NB staging tile
+constexpr int TPB = 256;        // threads of the tile kernels (4 waves)
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ inline double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// lower-triangular tile index t -> (i, j), i >= j, row-major over i
+__device__ inline void tri_decode(int t, int& i, int& j) {
+  int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((r + 1) * (r + 2) / 2 <= t) ++r;
+  while (r * (r + 1) / 2 > t) --r;
+  i = r;
+  j = t - r * (r + 1) / 2;
+}
+
+__device__ inline bool slot_live(const SlotParams& P, const SlotOut& O) {
+  return P.active && O.status != ST_CHOL_FAILED && O.status != ST_INACTIVE;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA tile machinery: a 64 x 64 output tile per 256-thread workgroup, each
+// wave owning a 32 x 32 quarter as 2 x 2 blocks of V_MFMA_F64_16X16X4_F64.
+// Operands are staged in LDS as [k][row] (row contiguous).  The instruction
+// computes D = P Q with P (16 x 4) given as lane l -> P(l & 15, l >> 4),
+// Q (4 x 16) as lane l -> Q(l >> 4, l & 15), and returns D(l >> 4 + 4e, l & 15)
+// in register e of lane l (checked on gfx950).  We feed P = B^T and Q = A^T,
+// so D = (A B)^T and lane l, register e holds C(row = l & 15, col = l >> 4 + 4e):
+// sixteen consecutive rows of one column per lane group, i.e. coalesced
+// column-major loads and stores of the C tile.
+// ---------------------------------------------------------------------------
+
+// Stage a KC x NB chunk of op(M) into dst[kk * LDSS + ii] = op(M)(i0 + ii, k0 + kk).
+//   TRANS = false: op(M)(i, k) = M[k * ld + i]   (i contiguous in memory)
+//   TRANS = true : op(M)(i, k) = M[i * ld + k]   (k contiguous in memory)
+template <bool TRANS>
+__device__ inline void stage(double* __restrict__ dst, const double* __restrict__ M, int ld, int i0, int k0,
+                             int tid) {
+  if (!TRANS) {
+    const int ii = 2 * (tid & 31);
+#pragma unroll
+    for (int r = 0; r < KC / 8; ++r) {
+      const int kk = (tid >> 5) + 8 * r;
+      const double2 v = *reinterpret_cast<const double2*>(M + (size_t)(k0 + kk) * ld + i0 + ii);
+      dst[kk * LDSS + ii] = v.x;
+      dst[kk * LDSS + ii + 1] = v.y;
+    }
+  } else {
+    const int kk = 2 * (tid & 15);
+#pragma unroll
+    for (int r = 0; r < NB / 16; ++r) {
+      const int ii = (tid >> 4) + 16 * r;
+      const double2 v = *reinterpret_cast<const double2*>(M + (size_t)(i0 + ii) * ld + k0 + kk);
+      dst[kk * LDSS + ii] = v.x;
+      dst[(kk + 1) * LDSS + ii] = v.y;
+    }
+  }
+}
+
+template <bool NEG>
+__device__ inline void mma_chunk(const double* __restrict__ sA, const double* __restrict__ sB, d4 (&acc)[2][2],
+                                 int lane, int wr, int wc) {
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int k0 = 0; k0 < KC; k0 += 4) {
+    const double* pa = sA + (k0 + kq) * LDSS + wr * 32 + r;
+    const double* pb = sB + (k0 + kq) * LDSS + wc * 32 + r;
+    double a0 = pa[0], a1 = pa[16];
+    const double b0 = pb[0], b1 = pb[16];
+    if (NEG) {
+      a0 = -a0;
+      a1 = -a1;
+    }
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+  }
+}
+
+__device__ inline void acc_zero(d4 (&acc)[2][2]) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+// row / col inside the 64 x 64 tile of accumulator element (bi, bj, e)
+__device__ inline int acc_row(int lane, int wr, int bi) { return wr * 32 + bi * 16 + (lane & 15); }
+__device__ inline int acc_col(int lane, int wc, int bj, int e) { return wc * 32 + bj * 16 + (lane >> 4) + 4 * e; }
+
+// C tile (column-major, leading dim ld) <-> accumulators
+__device__ inline void acc_load(d4 (&acc)[2][2], const double* __restrict__ C, int ld, int lane, int wr, int wc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        acc[bi][bj][e] = C[(size_t)acc_col(lane, wc, bj, e) * ld + acc_row(lane, wr, bi)];
+}
+
+__device__ inline void acc_store(const d4 (&acc)[2][2], double* __restrict__ C, int ld, int lane, int wr, int wc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        C[(size_t)acc_col(lane, wc, bj, e) * ld + acc_row(lane, wr, bi)] = acc[bi][bj][e];
+}
+
+// store the transpose: element (row, col) of the tile goes to C[row * ld + col]
+__device__ inline void acc_store_t(const d4 (&acc)[2][2], double* __restrict__ C, int ld, int lane, int wr,
+                                   int wc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        C[(size_t)acc_row(lane, wr, bi) * ld + acc_col(lane, wc, bj, e)] = acc[bi][bj][e];
+}
+
+// ---------------------------------------------------------------------------
+// kernel functions k(dx; theta) and d ln k / d theta_k, restated from
+// src/lib/Covariance.cpp:24-179 (gauss, exp, matern3_2, matern5_2).  The
+// Matern forms use the exact identity exp(-sum(u - log1p(q(u)))) =
+// exp(-sum u) * prod q'(u) with q' = 1 + q, one exp per pair instead of one
+// exp and d log1p.
+// ---------------------------------------------------------------------------
+constexpr double SQRT3 = 1.7320508075688772;
+constexpr double SQRT5 = 2.2360679774997896;
+
+// rho(dx) for the kernel; it[k] = 1 / theta_k (entries k >= d hold dx = 0)
+template <int DT>
+__device__ inline double kern_rho(int kern, const double* dx, const double* it) {
+  double s = 0.0;
+  if (kern == K_GAUSS) {
+#pragma unroll
+    for (int k = 0; k < DT; ++k) {
+      const double v = dx[k] * it[k];
+      s = fma(v, v, s);
+    }
+    return exp(-0.5 * s);
+  } else if (kern == K_EXP) {
+#pragma unroll
+    for (int k = 0; k < DT; ++k) s += fabs(dx[k] * it[k]);
+    return exp(-s);
+  } else if (kern == K_MATERN32) {
+    double pr = 1.0;
+#pragma unroll
+    for (int k = 0; k < DT; ++k) {
+      const double u = SQRT3 * fabs(dx[k] * it[k]);
+      s += u;
+      pr *= 1.0 + u;
+    }
+    return exp(-s) * pr;
+  } else {
+    double pr = 1.0;
+#pragma unroll
+    for (int k = 0; k < DT; ++k) {
+      const double u = SQRT5 * fabs(dx[k] * it[k]);
+      s += u;
+      pr *= 1.0 + u + (u * u) * (1.0 / 3.0);
+    }
+    return exp(-s) * pr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (1) covariance build: lower 64 x 64 tiles (diagonal tiles in full)
+// grid (ntiles(nblk_max), nslots), 256 threads
+// ---------------------------------------------------------------------------
+template <int DT>
+__global__ __launch_bounds__(TPB) void k_build(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
+                                               SlotOut* __restrict__ out) {
+  const int b = blockIdx.y;
+  const Slot S = slots[b];
+  const SlotParams& P = params[b];
+  if (!P.active) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[b].status = ST_INACTIVE;
+    return;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[b].status = ST_OK;
+  const int ntiles = S.nblk * (S.nblk + 1) / 2;
+  if ((int)blockIdx.x >= ntiles) return;
+  int ti, tj;
+  tri_decode(blockIdx.x, ti, tj);
+  const int i0 = ti * NB, j0 = tj * NB;
+  const int n = S.n, d = S.d, tid = threadIdx.x;
+
+  __shared__ double sXi[DMAX * NB];
+  __shared__ double sXj[DMAX * NB];
+  for (int q = tid; q < d * NB; q += TPB) {
+    const int k = q / NB, r = q % NB;
+    sXi[q] = (i0 + r < n) ? S.X[(size_t)k * n + i0 + r] : 0.0;
+    sXj[q] = (j0 + r < n) ? S.X[(size_t)k * n + j0 + r] : 0.0;
+  }
+  double it[DT];
+#pragma unroll
+  for (int k = 0; k < DT; ++k) it[k] = (k < d) ? 1.0 / P.theta[k] : 1.0;
+  const double factor = (S.noise == NOISE_NUGGET) ? P.alpha : 1.0;
+  // diagonal: 1 plus the cumulative jitter of the retry ladder, added in the
+  // same order as LinearAlgebra.cpp:88 (X.diag() += num_nugget * 10^inc)
+  double dval = 1.0;
+  {
+    double p10 = 1.0;
+    for (int q = 0; q < P.inc; ++q) {
+      dval += P.num_nugget * p10;
+      p10 *= 10.0;
+    }
+  }
+  __syncthreads();
+
+  const int ii = tid & 63;
+  const int gi = i0 + ii;
+  double dx[DT];
+  for (int r = 0; r < NB / 4; ++r) {
+    const int jj = (tid >> 6) + 4 * r;
+    const int gj = j0 + jj;
+    double v;
+    if (gi >= n || gj >= n) {
+      v = (gi == gj) ? 1.0 : 0.0;
+    } else if (gi == gj) {
+      v = dval;
+    } else {
+#pragma unroll
+      for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? sXj[k * NB + jj] - sXi[k * NB + ii] : 0.0;
+      v = kern_rho<DT>(S.kernel, dx, it) * factor;
+    }
+    S.A[(size_t)gj * S.npad + gi] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (2a) Cholesky of the diagonal block k and its inverse (one wave per slot).
+// Lane l holds row l of the block in registers; pivots and columns are
+// broadcast with v_readlane.  Writes L_kk to A (zero strict upper part) and
+// L_kk^-1 to the same block of W, the per-block partial log-det, and flags a
+// non-positive / NaN pivot (dpotrf failure, LinearAlgebra.cpp:71).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_potrf_diag(const Slot* __restrict__ slots,
+                                                   const SlotParams* __restrict__ params,
+                                                   SlotOut* __restrict__ out, int k) {
+  const int b = blockIdx.x;
+  const Slot S = slots[b];
+  if (!slot_live(params[b], out[b]) || k >= S.nblk) return;
+  const int l = threadIdx.x;
+  const int k0 = k * NB;
+  const int ld = S.npad;
+  double* Ab = S.A + (size_t)k0 * ld + k0;
+  double* Wb = S.W + (size_t)k0 * ld + k0;
+
+  double a[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) a[c] = Ab[(size_t)c * ld + l];
+
+  __shared__ double sL[NB * (NB + 1)];
+  __shared__ double sDiag[NB];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const double ajj = readlane_d(a[j], j);
+    ok = ok && (ajj > 0.0);
+    const double ljj = sqrt(ajj);
+    const double rinv = 1.0 / ljj;
+    a[j] = (l == j) ? ljj : ((l > j) ? a[j] * rinv : 0.0);
+#pragma unroll
+    for (int c = j + 1; c < NB; ++c) {
+      const double lcj = readlane_d(a[j], c);
+      a[c] = fma(-a[j], lcj, a[c]);
+    }
+    if (l == 0) sDiag[j] = ljj;
+  }
+  if (!ok) {
+    if (l == 0) out[b].status = ST_CHOL_FAILED;
+    return;
+  }
+  // L_kk -> A (lower, zero above) and LDS (column-major, stride NB + 1)
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    const double v = (l >= c) ? a[c] : 0.0;
+    Ab[(size_t)c * ld + l] = v;
+    sL[c * (NB + 1) + l] = v;
+  }
+  __syncthreads();
+  // log-det partial
+  {
+    const double ld_part = wave_sum(log(sDiag[l]));
+    if (l == 0) S.red[RED_LOGDET + k] = ld_part;
+  }
+  // row l of L_kk^-1: X(l, c) = (delta_lc - sum_{m > c} X(l, m) L(m, c)) / L(c, c)
+  double x[NB];
+#pragma unroll
+  for (int c = NB - 1; c >= 0; --c) {
+    double s = (l == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int m = c + 1; m < NB; ++m) s = fma(-x[m], sL[c * (NB + 1) + m], s);
+    x[c] = s / sL[c * (NB + 1) + c];
+  }
+#pragma unroll
+  for (int c = 0; c < NB; ++c) Wb[(size_t)c * ld + l] = x[c];
+}
+
+// ---------------------------------------------------------------------------
+// (2b) panel: L(r, k) = A(r, k) * L_kk^-T for row blocks r > k
+// grid (nblk_max - k - 1, nslots), 256 threads
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ slots,
+                                                     const SlotParams* __restrict__ params,
+                                                     SlotOut* __restrict__ out, int k) {
+  const int b = blockIdx.y;
+  const Slot S = slots[b];
+  const int r = k + 1 + blockIdx.x;
+  if (!slot_live(params[b], out[b]) || r >= S.nblk) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  const int i0 = r * NB, k0 = k * NB, ld = S.npad;
+  __shared__ double sA[KC * LDSS];
+  __shared__ double sB[KC * LDSS];
+  d4 acc[2][2];
+  acc_zero(acc);
+  for (int kc = 0; kc < NB; kc += KC) {
+    stage<false>(sA, S.A, ld, i0, k0 + kc, tid);  // A(i0 + ii, k0 + kc + kk)
+    stage<false>(sB, S.W, ld, k0, k0 + kc, tid);  // Dinv(c, kc + kk) at W[(k0+kc+kk)*ld + k0 + c]
+    __syncthreads();
+    mma_chunk<false>(sA, sB, acc, lane, wr, wc);
+    __syncthreads();
+  }
+  acc_store(acc, S.A + (size_t)k0 * ld + i0, ld, lane, wr, wc);
+}
+
+// ---------------------------------------------------------------------------
+// (2c) trailing update: A(i, j) -= L(i, k) L(j, k)^T for k < j <= i
+// grid (T(T+1)/2 with T = nblk_max - k - 1, nslots), 256 threads
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_potrf_update(const Slot* __restrict__ slots,
+                                                      const SlotParams* __restrict__ params,
+                                                      SlotOut* __restrict__ out, int k) {
+  const int b = blockIdx.y;
+  const Slot S = slots[b];
+  if (!slot_live(params[b], out[b])) return;
+  const int T = S.nblk - k - 1;  // trailing blocks of this slot (negative past its end)
+  if (T <= 0 || (int)blockIdx.x >= T * (T + 1) / 2) return;
+  int ti, tj;
+  tri_decode(blockIdx.x, ti, tj);
+  const int i0 = (k + 1 + ti) * NB, j0 = (k + 1 + tj) * NB, k0 = k * NB, ld = S.npad;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  __shared__ double sA[KC * LDSS];
+  __shared__ double sB[KC * LDSS];
+  d4 acc[2][2];
+  double* C = S.A + (size_t)j0 * ld + i0;
+  acc_load(acc, C, ld, lane, wr, wc);
+  for (int kc = 0; kc < NB; kc += KC) {
+    stage<false>(sA, S.A, ld, i0, k0 + kc, tid);
+    stage<false>(sB, S.A, ld, j0, k0 + kc, tid);
+    __syncthreads();
+    mma_chunk<true>(sA, sB, acc, lane, wr, wc);
+    __syncthreads();
+  }
+  acc_store(acc, C, ld, lane, wr, wc);
+}
+
+// ---------------------------------------------------------------------------
+// (3) L^-1 by recursive doubling.  At level s (block size s tiles), for each
+// aligned pair (top = 2ps, bot = top + s):
+//   step 0:  T = L(bot, top) * Linv(top, top)          -> W upper (transposed)
+//   step 1:  Linv(bot, top) = -Linv(bot, bot) * T      -> W lower
+// grid (npairs_max * s * s, nslots), 256 threads
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_trtri(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
+                                               SlotOut* __restrict__ out, int s, int step) {
+  const int b = blockIdx.y;
+  const Slot S = slots[b];
+  if (!slot_live(params[b], out[b])) return;
+  const int per_pair = s * s;
+  const int pr = blockIdx.x / per_pair;
+  const int rem = blockIdx.x % per_pair;
+  const int ti = rem / s, tj = rem % s;
+  const int top = 2 * pr * s, bot = top + s;
+  const int it = bot + ti, jt = top + tj;  // tile row / col of the output
+  if (bot >= S.nblk || it >= S.nblk) return;
+  const int ld = S.npad;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  const int i0 = it * NB, j0 = jt * NB;
+  __shared__ double sA[KC * LDSS];
+  __shared__ double sB[KC * LDSS];
+  d4 acc[2][2];
+  acc_zero(acc);
+  if (step == 0) {
+    // T(i, j) = sum_{kt = jt}^{top + s - 1} L(i, kt) Linv(kt, j)
+    for (int kt = jt; kt < top + s; ++kt)
+      for (int kc = 0; kc < NB; kc += KC) {
+        stage<false>(sA, S.A, ld, i0, kt * NB + kc, tid);
+        stage<true>(sB, S.W, ld, j0, kt * NB + kc, tid);
+        __syncthreads();
+        mma_chunk<false>(sA, sB, acc, lane, wr, wc);
+        __syncthreads();
+      }
+    // T(i, j) stored at W[(i0 + row) * ld + j0 + col] (strict upper triangle of W)
+    acc_store_t(acc, S.W + (size_t)i0 * ld + j0, ld, lane, wr, wc);
+  } else {
+    // Linv(i, j) = -sum_{kt = bot}^{it} Linv(i, kt) T(kt, j)
+    for (int kt = bot; kt <= it; ++kt)
+      for (int kc = 0; kc < NB; kc += KC) {
+        stage<false>(sA, S.W, ld, i0, kt * NB + kc, tid);
+        stage<false>(sB, S.W, ld, j0, kt * NB + kc, tid);
+        __syncthreads();
+        mma_chunk<true>(sA, sB, acc, lane, wr, wc);
+        __syncthreads();
+      }
+    acc_store(acc, S.W + (size_t)j0 * ld + i0, ld, lane, wr, wc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (4) triangular products with the explicit inverse.
+// V columns: [0, p) F*, p y*, p+1 r, p+2 E*, p+3 x, p+4.. p+6 estimator scratch
+// ---------------------------------------------------------------------------
+// out[:, c] = Linv * in[:, c] for rows of block rb (grid (nblk_max, nslots))
+// in columns come from `src` (n rows, ld = n, zero beyond n) ; out to V.
+__device__ inline void lower_matvec_block(const Slot& S, int rb, const double* __restrict__ src, int lds, int nsrc,
+                                          int ncols, double* __restrict__ dst, int ldd, double* __restrict__ part_sq) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int i0 = rb * NB, ld = S.npad, n = S.n;
+  const int kend = min(i0 + NB, n);  // columns k <= row, and k < n (src is zero beyond n)
+  __shared__ double red[4][4][NB];
+  double ssq = 0.0;
+  for (int c0 = 0; c0 < ncols; c0 += 4) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int nc = min(4, ncols - c0);
+    for (int k = w; k < kend; k += 4) {
+      const double lv = S.W[(size_t)k * ld + i0 + lane];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c < nc) acc[c] = fma(lv, src[(size_t)(c0 + c) * lds + k], acc[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) red[w][c][lane] = acc[c];
+    __syncthreads();
+    if (w == 0) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c >= nc) continue;
+        // rows above the diagonal read W's upper part of the diagonal block,
+        // which is exactly zero; rows >= n are identity rows times zero input.
+        const double v = red[0][c][lane] + red[1][c][lane] + red[2][c][lane] + red[3][c][lane];
+        dst[(size_t)(c0 + c) * ldd + i0 + lane] = v;
+        ssq = fma(v, v, ssq);
+      }
+    }
+    __syncthreads();
+  }
+  (void)nsrc;
+  if (part_sq != nullptr && w == 0) {
+    const double t = wave_sum(ssq);
+    if (lane == 0) part_sq[rb] = t;
+  }
+}
+
+// [F* | y*] = Linv [F | y]
+__global__ __launch_bounds__(TPB) void k_solve_fy(const Slot* __restrict__ slots,
+                                                  const SlotParams* __restrict__ params,
+                                                  SlotOut* __restrict__ out) {
+  const int b = blockIdx.y;
+  const Slot S = slots[b];
+  if (!slot_live(params[b], out[b]) || (int)blockIdx.x >= S.nblk) return;
+  const int rb = blockIdx.x;
+  if (S.p > 0) lower_matvec_block(S, rb, S.F, S.n, S.n, S.p, S.V, S.npad, nullptr);
+  lower_matvec_block(S, rb, S.y, S.n, S.n, 1, S.V + (size_t)S.p * S.npad, S.npad, nullptr);
+}
+
+// GLS: R* = chol_upper(F*'F*), beta = R*^-1 R*^-T F*'y*, r = y - F beta
+// (KrigingImpl.cpp:105-121).  One workgroup per slot.
+__global__ __launch_bounds__(TPB) void k_gls(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
+                                             SlotOut* __restrict__ out) {
+  const int b = blockIdx.x;
+  const Slot S = slots[b];
+  if (!slot_live(params[b], out[b])) return;
+  const int tid = threadIdx.x, p = S.p, n = S.n, ld = S.npad;
+  __shared__ double G[PMAX * (PMAX + 1)];  // p x (p + 1): [F*'F* | F*'y*]
+  __shared__ double part[TPB];
+  __shared__ double beta[PMAX];
+  for (int a = 0; a < p; ++a)
+    for (int c = a; c <= p; ++c) {  // column c == p is F*'y*
+      double s = 0.0;
+      for (int i = tid; i < n; i += TPB) s = fma(S.V[(size_t)a * ld + i], S.V[(size_t)c * ld + i], s);
+      part[tid] = s;
+      __syncthreads();
+      for (int h = TPB / 2; h > 0; h >>= 1) {
+        if (tid < h) part[tid] += part[tid + h];
+        __syncthreads();
+      }
+      if (tid == 0) G[a + c * PMAX] = part[0];
+      __syncthreads();
+    }
+  if (tid == 0) {
+    // upper Cholesky U'U = G (stored in place in the upper triangle)
+    bool ok = true;
+    for (int j = 0; j < p; ++j) {
+      double s = G[j + j * PMAX];
+      for (int q = 0; q < j; ++q) s -= G[q + j * PMAX] * G[q + j * PMAX];
+      ok = ok && (s > 0.0);
+      const double ujj = sqrt(s);
+      G[j + j * PMAX] = ujj;
+      for (int c = j + 1; c < p; ++c) {
+        double t = G[j + c * PMAX];
+        for (int q = 0; q < j; ++q) t -= G[q + j * PMAX] * G[q + c * PMAX];
+        G[j + c * PMAX] = t / ujj;
+      }
+    }
+    // beta: U' z = h, U beta = z
+    double z[PMAX];
+    for (int a = 0; a < p; ++a) {
+      double t = G[a + p * PMAX];
+      for (int q = 0; q < a; ++q) t -= G[q + a * PMAX] * z[q];
+      z[a] = t / G[a + a * PMAX];
+    }
+    for (int a = p - 1; a >= 0; --a) {
+      double t = z[a];
+      for (int q = a + 1; q < p; ++q) t -= G[a + q * PMAX] * z[q];
+      z[a] = t / G[a + a * PMAX];
+    }
+    for (int a = 0; a < p; ++a) {
+      beta[a] = z[a];
+      out[b].beta[a] = z[a];
+    }
+    if (!ok) out[b].status = ST_CHOL_FAILED;
+  }
+  __syncthreads();
+  // residual r = y - F beta into V column p + 1 (zero in padded rows)
+  double* r = S.V + (size_t)(p + 1) * ld;
+  for (int i = tid; i < ld; i += TPB) {
+    double v = 0.0;
+    if (i < n) {
+      double s = 0.0;
+      for (int a = 0; a < p; ++a) s = fma(S.F[(size_t)a * n + i], beta[a], s);
+      v = S.y[i] - s;
+    }
+    r[i] = v;
+  }
+}
+
+// E* = Linv r, SSE partials
+__global__ __launch_bounds__(TPB) void k_solve_e(const Slot* __restrict__ slots,
+                                                 const SlotParams* __restrict__ params,
+                                                 SlotOut* __restrict__ out) {
+  const int b = blockIdx.y;
+  const Slot S = slots[b];
+  if (!slot_live(params[b], out[b]) || (int)blockIdx.x >= S.nblk) return;
+  const double* r = S.V + (size_t)(S.p + 1) * S.npad;
+  double* e = S.V + (size_t)(S.p + 2) * S.npad;
+  lower_matvec_block(S, blockIdx.x, r, S.npad, S.n, 1, e, S.npad, S.red + red_sse_off(S.nblk));
+}
+
+// x = Linv^T E*, and the 1-norms (max column abs sums) of L and Linv
+__global__ __launch_bounds__(TPB) void k_solve_x(const Slot* __restrict__ slots,
+                                                 const SlotParams* __restrict__ params,
+                                                 SlotOut* __restrict__ out) {
+  const int b = blockIdx.y;
+  const Slot S = slots[b];
+  if (!slot_live(params[b], out[b]) || (int)blockIdx.x >= S.nblk) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, ld = S.npad, n = S.n;
+  const double* e = S.V + (size_t)(S.p + 2) * ld;
+  double* x = S.V + (size_t)(S.p + 3) * ld;
+  double mW = 0.0, mL = 0.0;
+  for (int cc = w; cc < NB; cc += 4) {
+    const int c = blockIdx.x * NB + cc;
+    double s = 0.0, aw = 0.0, al = 0.0;
+    for (int i = c + lane; i < ld; i += 64) {
+      const double wv = S.W[(size_t)c * ld + i];
+      s = fma(wv, e[i], s);
+      aw += fabs(wv);
+      al += fabs(S.A[(size_t)c * ld + i]);
+    }
+    s = wave_sum(s);
+    aw = wave_sum(aw);
+    al = wave_sum(al);
+    if (lane == 0) x[c] = s;
+    if (c < n) {
+      mW = fmax(mW, aw);
+      mL = fmax(mL, al);
+    }
+  }
+  __shared__ double sm[2][4];
+  if (lane == 0) {
+    sm[0][w] = mW;
+    sm[1][w] = mL;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    S.red[red_colW_off(S.nblk) + blockIdx.x] = fmax(fmax(sm[0][0], sm[0][1]), fmax(sm[0][2], sm[0][3]));
+    S.red[red_colL_off(S.nblk) + blockIdx.x] = fmax(fmax(sm[1][0], sm[1][1]), fmax(sm[1][2], sm[1][3]));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (5) LL value and condition check (Kriging.cpp:247-289,
+// LinearAlgebra.cpp:68-115).  One thread per slot.
+// ---------------------------------------------------------------------------
+__global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
+                              SlotOut* __restrict__ out, int nslots) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nslots) return;
+  const Slot S = slots[b];
+  const SlotParams& P = params[b];
+  SlotOut& O = out[b];
+  if (!slot_live(P, O)) return;
+  const int nb = S.nblk;
+  double logdet = 0.0, sse = 0.0, nL = 0.0, nW = 0.0;
+  for (int q = 0; q < nb; ++q) {
+    logdet += S.red[RED_LOGDET + q];
+    sse += S.red[red_sse_off(nb) + q];
+    nL = fmax(nL, S.red[red_colL_off(nb) + q]);
+    nW = fmax(nW, S.red[red_colW_off(nb) + q]);
+  }
+  const double n = (double)S.n;
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  O.logdet = logdet;
+  O.sse = sse;
+  O.norm1_L = nL;
+  O.norm1_Linv = nW;
+  const double rc = (1.0 / nL) / nW;
+  O.rcond2 = rc * rc;
+  if (P.rcond_check && !(O.rcond2 >= P.min_rcond) && O.status == ST_OK) O.status = ST_RCOND_SUSPECT;
+  double ll, s2g;
+  if (S.noise == NOISE_NUGGET) {
+    const double a = P.alpha;
+    double s2 = P.sigma2, nug = P.nugget;
+    if (P.est_sigma2) {
+      if (P.est_nugget) {
+        const double var = sse / n;
+        s2 = a * var;
+        nug = (1.0 - a) * var;
+      } else {
+        s2 = P.nugget * a / (1.0 - a);
+      }
+    } else if (P.est_nugget) {
+      nug = P.sigma2 * (1.0 - a) / a;
+    }
+    const double tv = s2 + nug;
+    ll = -0.5 * (n * log(two_pi * tv) + 2.0 * logdet + sse / tv);
+    s2g = tv;
+  } else if (P.est_sigma2) {
+    s2g = sse / n;
+    ll = -0.5 * (n * log(two_pi * s2g) + 2.0 * logdet + n);
+  } else {
+    s2g = P.sigma2;
+    ll = -0.5 * (n * log(two_pi * s2g) + 2.0 * logdet + sse / s2g);
+  }
+  O.ll = ll;
+  O.sigma2 = s2g;
+}
+
+// ---------------------------------------------------------------------------
+// (6) fused R^-1 tile + adjoint contraction.  Tile (i >= j) of
+// R^-1 = Linv^T Linv is accumulated on MFMA; then every strictly-lower
+// element (gi > gj) contributes, with dR_ij/dtheta_k = R_ij g_k(x_j - x_i)
+// recomputed from X (KrigingImpl.cpp:871-884, Kriging.cpp:303-316):
+//   s1_k += x_i x_j R_ij g_k      s2_k += Rinv_ij R_ij g_k
+//   a1   += x_i x_j rho_ij        a2   += Rinv_ij rho_ij
+// and diagonal elements add x_i^2 and Rinv_ii.  Partials per tile go to
+// red[grad_off + tile * GRAD_STRIDE]: [s1 (DT) | s2 (DT) | a1 a2 dg1 dg2].
+// grid (nblk_max (nblk_max + 1) / 2, nslots), 256 threads
+// ---------------------------------------------------------------------------
+template <int DT>
+__global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ slots,
+                                                       const SlotParams* __restrict__ params,
+                                                       SlotOut* __restrict__ out) {
+  const int b = blockIdx.y;
+  const Slot S = slots[b];
+  const SlotParams& P = params[b];
+  if (!slot_live(P, out[b]) || !P.want_grad) return;
+  const int ntiles = S.nblk * (S.nblk + 1) / 2;
+  if ((int)blockIdx.x >= ntiles) return;
+  int it, jt;
+  tri_decode(blockIdx.x, it, jt);
+  const int ld = S.npad, n = S.n, d = S.d;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  const int i0 = it * NB, j0 = jt * NB;
+  __shared__ double sA[KC * LDSS];
+  __shared__ double sB[KC * LDSS];
+  d4 acc[2][2];
+  acc_zero(acc);
+  for (int kt = it; kt < S.nblk; ++kt)
+    for (int kc = 0; kc < NB; kc += KC) {
+      stage<true>(sA, S.W, ld, i0, kt * NB + kc, tid);  // Linv(kt*NB + kc + kk, i0 + ii)
+      stage<true>(sB, S.W, ld, j0, kt * NB + kc, tid);  // Linv(kt*NB + kc + kk, j0 + jj)
+      __syncthreads();
+      mma_chunk<false>(sA, sB, acc, lane, wr, wc);
+      __syncthreads();
+    }
+
+  // epilogue: X rows of both blocks and the x vector in LDS (reuse sA / sB)
+  double* sXi = sA;  // d x NB
+  double* sXj = sB;
+  __shared__ double sxi[NB], sxj[NB];
+  const double* xv = S.V + (size_t)(S.p + 3) * ld;
+  for (int q = tid; q < d * NB; q += TPB) {
+    const int k = q / NB, r = q % NB;
+    sXi[q] = (i0 + r < n) ? S.X[(size_t)k * n + i0 + r] : 0.0;
+    sXj[q] = (j0 + r < n) ? S.X[(size_t)k * n + j0 + r] : 0.0;
+  }
+  if (tid < NB) {
+    sxi[tid] = xv[i0 + tid];
+    sxj[tid] = xv[j0 + tid];
+  }
+  __syncthreads();
+
+  double it_[DT], it2[DT];
+#pragma unroll
+  for (int k = 0; k < DT; ++k) {
+    const double t = (k < d) ? P.theta[k] : 1.0;
+    it_[k] = 1.0 / t;
+    it2[k] = it_[k] * it_[k];
+  }
+  const double factor = (S.noise == NOISE_NUGGET) ? P.alpha : 1.0;
+  const int kern = S.kernel;
+  double s1[DT], s2[DT];
+#pragma unroll
+  for (int k = 0; k < DT; ++k) {
+    s1[k] = 0.0;
+    s2[k] = 0.0;
+  }
+  double a1 = 0.0, a2 = 0.0, dg1 = 0.0, dg2 = 0.0;
+
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = acc_row(lane, wr, bi), c = acc_col(lane, wc, bj, e);
+        const int gi = i0 + r, gj = j0 + c;
+        const double rinv = acc[bi][bj][e];
+        if (gi >= n || gj > gi) continue;
+        const double xx = sxi[r] * sxj[c];
+        if (gi == gj) {
+          dg1 += xx;
+          dg2 += rinv;
+          continue;
+        }
+        double dx[DT];
+        double s = 0.0, pr = 1.0;
+#pragma unroll
+        for (int k = 0; k < DT; ++k) {
+          dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
+        }
+        double g[DT];
+        double rho;
+        if (kern == K_GAUSS) {
+#pragma unroll
+          for (int k = 0; k < DT; ++k) {
+            const double v = dx[k] * it_[k];
+            s = fma(v, v, s);
+            g[k] = (dx[k] * dx[k]) * (it2[k] * it_[k]);
+          }
+          rho = exp(-0.5 * s);
+        } else if (kern == K_EXP) {
+#pragma unroll
+          for (int k = 0; k < DT; ++k) {
+            s += fabs(dx[k] * it_[k]);
+            g[k] = fabs(dx[k]) * it2[k];
+          }
+          rho = exp(-s);
+        } else if (kern == K_MATERN32) {
+#pragma unroll
+          for (int k = 0; k < DT; ++k) {
+            const double u = SQRT3 * fabs(dx[k] * it_[k]);
+            s += u;
+            pr *= 1.0 + u;
+            g[k] = (u * u) / (1.0 + u) * it_[k];
+          }
+          rho = exp(-s) * pr;
+        } else {
+#pragma unroll
+          for (int k = 0; k < DT; ++k) {
+            const double u = SQRT5 * fabs(dx[k] * it_[k]);
+            const double aa = 1.0 + u;
+            const double bb = (u * u) * (1.0 / 3.0);
+            s += u;
+            pr *= aa + bb;
+            g[k] = (aa * bb) / (aa + bb) * it_[k];
+          }
+          rho = exp(-s) * pr;
+        }
+        const double Rij = rho * factor;
+#pragma unroll
+        for (int k = 0; k < DT; ++k) {
+          const double gk = Rij * g[k];
+          s1[k] = fma(xx, gk, s1[k]);
+          s2[k] = fma(rinv, gk, s2[k]);
+        }
+        a1 = fma(xx, rho, a1);
+        a2 = fma(rinv, rho, a2);
+      }
+
+  // workgroup reduction of the 2 * DT + 4 partials
+  __shared__ double sred[4][2 * DT + 4];
+  double* dst = S.red + red_grad_off(S.nblk) + (size_t)blockIdx.x * GRAD_STRIDE;
+#pragma unroll
+  for (int k = 0; k < DT; ++k) {
+    const double v1 = wave_sum(s1[k]);
+    const double v2 = wave_sum(s2[k]);
+    if (lane == 0) {
+      sred[w][k] = v1;
+      sred[w][DT + k] = v2;
+    }
+  }
+  {
+    const double v0 = wave_sum(a1), v1 = wave_sum(a2), v2 = wave_sum(dg1), v3 = wave_sum(dg2);
+    if (lane == 0) {
+      sred[w][2 * DT + 0] = v0;
+      sred[w][2 * DT + 1] = v1;
+      sred[w][2 * DT + 2] = v2;
+      sred[w][2 * DT + 3] = v3;
+    }
+  }
+  __syncthreads();
+  if (tid < 2 * DT + 4) {
+    const double v = sred[0][tid] + sred[1][tid] + sred[2][tid] + sred[3][tid];
+    int q;
+    if (tid < DT) q = tid;                       // s1
+    else if (tid < 2 * DT) q = DMAX + tid - DT;  // s2
+    else q = 2 * DMAX + (tid - 2 * DT);          // a1 a2 dg1 dg2
+    dst[q] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (7) gradient (Kriging.cpp:303-326): sums of the tile partials in tile order
+// ---------------------------------------------------------------------------
+__global__ void k_finalize_grad(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
+                                SlotOut* __restrict__ out, int nslots) {
+  const int b = blockIdx.x;
+  if (b >= nslots) return;
+  const Slot S = slots[b];
+  const SlotParams& P = params[b];
+  SlotOut& O = out[b];
+  if (!slot_live(P, O) || !P.want_grad) return;
+  const int q = threadIdx.x;  // 0 .. GRAD_STRIDE-1
+  const int ntiles = S.nblk * (S.nblk + 1) / 2;
+  __shared__ double tot[GRAD_STRIDE];
+  if (q < GRAD_STRIDE) {
+    double s = 0.0;
+    const double* src = S.red + red_grad_off(S.nblk) + q;
+    for (int t = 0; t < ntiles; ++t) s += src[(size_t)t * GRAD_STRIDE];
+    tot[q] = s;
+  }
+  __syncthreads();
+  if (q != 0) return;
+  const int d = S.d;
+  const double s2g = O.sigma2;
+  for (int k = 0; k < d; ++k) {
+    // term1 = 2 s1, term2 = -2 s2, grad = (term1 / sigma2 + term2) / 2
+    O.grad[k] = tot[k] / s2g - tot[DMAX + k];
+  }
+  if (P.gamma_has_alpha && S.noise == NOISE_NUGGET) {
+    const double a1 = tot[2 * DMAX], a2 = tot[2 * DMAX + 1], dg1 = tot[2 * DMAX + 2], dg2 = tot[2 * DMAX + 3];
+    const double a = P.alpha;
+    double ga;
+    if (P.est_sigma2 && P.est_nugget) {
+      // -0.5 * (-(x' D x) / var + <Rinv, D>), D = R / alpha with zero diagonal
+      ga = -0.5 * (-(2.0 * a1) / s2g + 2.0 * a2);
+    } else if (P.est_sigma2 && !P.est_nugget) {
+      const double tv = s2g;
+      const double q1 = 2.0 * a1 + dg1;
+      const double tr = (2.0 * a2 + dg2) / tv;
+      ga = -0.5 * (-q1 / (tv * tv) + tr) * P.nugget / (1.0 - a) / (1.0 - a);
+    } else {
+      ga = 0.0;
+    }
+    O.grad[d] = ga;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (8) 1-norm estimate of Linv (Higham, ACM TOMS 14 (1988) 381-396, Alg. 4.1
+// with the alternating-sign extra estimate), run only for slots whose exact
+// rcond fell below the threshold.  The estimate never exceeds the exact norm,
+// so an exact rcond above the threshold implies the estimate-based one is
+// too.  One workgroup per slot; uses V columns p+4 .. p+6 as scratch.
+// ---------------------------------------------------------------------------
+__device__ void est_lower_mv(const Slot& S, const double* __restrict__ in, double* __restrict__ o) {
+  // o = Linv * in  (row i: sum_{k <= i} W[k*ld + i] in[k]) ; thread per row
+  const int ld = S.npad, n = S.n;
+  for (int i = threadIdx.x; i < n; i += TPB) {
+    double s = 0.0;
+    for (int k = 0; k <= i; ++k) s = fma(S.W[(size_t)k * ld + i], in[k], s);
+    o[i] = s;
+  }
+}
+__device__ void est_upper_mv(const Slot& S, const double* __restrict__ in, double* __restrict__ o) {
+  // o = Linv^T * in  (column c: sum_{i >= c} W[c*ld + i] in[i]) ; wave per column
+  const int ld = S.npad, n = S.n, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int c = w; c < n; c += TPB / 64) {
+    double s = 0.0;
+    for (int i = c + lane; i < n; i += 64) s = fma(S.W[(size_t)c * ld + i], in[i], s);
+    s = wave_sum(s);
+    if (lane == 0) o[c] = s;
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_rcond_estimate(const Slot* __restrict__ slots,
+                                                        const SlotParams* __restrict__ params,
+                                                        SlotOut* __restrict__ out) {
+  const int b = blockIdx.x;
+  const Slot S = slots[b];
+  const SlotParams& P = params[b];
+  SlotOut& O = out[b];
+  if (!P.active || O.status != ST_RCOND_SUSPECT) return;
+  const int n = S.n, ld = S.npad, tid = threadIdx.x;
+  double* x = S.V + (size_t)(S.p + 4) * ld;
+  double* xi = S.V + (size_t)(S.p + 5) * ld;
+  double* y = S.V + (size_t)(S.p + 6) * ld;
+  __shared__ double sh[TPB];
+  __shared__ int shi[TPB];
+  __shared__ double sest, sestold;
+  __shared__ int sj, sflag;
+
+  auto block_sum_abs = [&](const double* v) -> double {
+    double s = 0.0;
+    for (int i = tid; i < n; i += TPB) s += fabs(v[i]);
+    sh[tid] = s;
+    __syncthreads();
+    for (int h = TPB / 2; h > 0; h >>= 1) {
+      if (tid < h) sh[tid] += sh[tid + h];
+      __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+  };
+  auto block_argmax_abs = [&](const double* v) -> int {
+    double best = -1.0;
+    int bi = 0;
+    for (int i = tid; i < n; i += TPB)
+      if (fabs(v[i]) > best) {
+        best = fabs(v[i]);
+        bi = i;
+      }
+    sh[tid] = best;
+    shi[tid] = bi;
+    __syncthreads();
+    for (int h = TPB / 2; h > 0; h >>= 1) {
+      if (tid < h) {
+        const double o = sh[tid + h];
+        const int oi = shi[tid + h];
+        if (o > sh[tid] || (o == sh[tid] && oi < shi[tid])) {
+          sh[tid] = o;
+          shi[tid] = oi;
+        }
+      }
+      __syncthreads();
+    }
+    const int r = shi[0];
+    __syncthreads();
+    return r;
+  };
+
+  double est;
+  for (int i = tid; i < n; i += TPB) x[i] = 1.0 / n;
+  __syncthreads();
+  est_lower_mv(S, x, y);
+  __syncthreads();
+  if (n == 1) {
+    est = fabs(y[0]);
+  } else {
+    est = block_sum_abs(y);
+    for (int i = tid; i < n; i += TPB) xi[i] = (y[i] >= 0.0) ? 1.0 : -1.0;
+    __syncthreads();
+    est_upper_mv(S, xi, x);
+    __syncthreads();
+    int j = block_argmax_abs(x);
+    int iter = 2;
+    for (;;) {
+      for (int i = tid; i < n; i += TPB) x[i] = (i == j) ? 1.0 : 0.0;
+      __syncthreads();
+      est_lower_mv(S, x, y);
+      __syncthreads();
+      const double estold = est;
+      est = block_sum_abs(y);
+      if (tid == 0) sflag = 0;
+      __syncthreads();
+      for (int i = tid; i < n; i += TPB)
+        if (((y[i] >= 0.0) ? 1.0 : -1.0) != xi[i]) sflag = 1;
+      __syncthreads();
+      const int changed = sflag;
+      __syncthreads();
+      if (!changed || est <= estold) break;
+      for (int i = tid; i < n; i += TPB) xi[i] = (y[i] >= 0.0) ? 1.0 : -1.0;
+      __syncthreads();
+      est_upper_mv(S, xi, x);
+      __syncthreads();
+      const int jlast = j;
+      j = block_argmax_abs(x);
+      if (x[jlast] != fabs(x[j]) && iter < 5) {
+        ++iter;
+        continue;
+      }
+      break;
+    }
+    for (int i = tid; i < n; i += TPB) x[i] = ((i & 1) ? -1.0 : 1.0) * (1.0 + (double)i / (double)(n - 1));
+    __syncthreads();
+    est_lower_mv(S, x, y);
+    __syncthreads();
+    double temp = block_sum_abs(y);
+    temp = 2.0 * (temp / (double)(3 * n));
+    if (temp > est) est = temp;
+  }
+  (void)sest;
+  (void)sestold;
+  (void)sj;
+  if (tid == 0) {
+    const double rc = (1.0 / O.norm1_L) / est;
+    O.rcond2 = rc * rc;
+    O.status = (O.rcond2 < P.min_rcond) ? ST_RCOND_LOW : ST_OK;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int tri(int t) { return t * (t + 1) / 2; }
+
+hipError_t launch_build(const Batch& B) {
+  dim3 grid(tri(B.nblk_max), B.nslots);
+  if (B.dmax <= 4)
+    hipLaunchKernelGGL(k_build<4>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  else if (B.dmax <= 8)
+    hipLaunchKernelGGL(k_build<8>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  else if (B.dmax <= 16)
+    hipLaunchKernelGGL(k_build<16>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  else
+    hipLaunchKernelGGL(k_build<32>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  return hipGetLastError();
+}
+hipError_t launch_potrf_diag(const Batch& B, int k) {
+  hipLaunchKernelGGL(k_potrf_diag, dim3(B.nslots), dim3(64), 0, B.stream, B.slots, B.params, B.out, k);
+  return hipGetLastError();
+}
+hipError_t launch_potrf_panel(const Batch& B, int k) {
+  const int T = B.nblk_max - k - 1;
+  if (T <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_potrf_panel, dim3(T, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, k);
+  return hipGetLastError();
+}
+hipError_t launch_potrf_update(const Batch& B, int k) {
+  const int T = B.nblk_max - k - 1;
+  if (T <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_potrf_update, dim3(tri(T), B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, k);
+  return hipGetLastError();
+}
+int trtri_levels(int nblk) {
+  int lv = 0;
+  while ((1 << lv) < nblk) ++lv;
+  return lv;
+}
+hipError_t launch_trtri_level(const Batch& B, int s, int step) {
+  const int npairs = (B.nblk_max + 2 * s - 1) / (2 * s);
+  if (npairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trtri, dim3(npairs * s * s, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, s,
+                     step);
+  return hipGetLastError();
+}
+hipError_t launch_solve_fy(const Batch& B) {
+  hipLaunchKernelGGL(k_solve_fy, dim3(B.nblk_max, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  return hipGetLastError();
+}
+hipError_t launch_gls(const Batch& B) {
+  hipLaunchKernelGGL(k_gls, dim3(B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  return hipGetLastError();
+}
+hipError_t launch_solve_e(const Batch& B) {
+  hipLaunchKernelGGL(k_solve_e, dim3(B.nblk_max, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  return hipGetLastError();
+}
+hipError_t launch_solve_x(const Batch& B) {
+  hipLaunchKernelGGL(k_solve_x, dim3(B.nblk_max, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  return hipGetLastError();
+}
+hipError_t launch_finalize_ll(const Batch& B) {
+  hipLaunchKernelGGL(k_finalize_ll, dim3((B.nslots + 63) / 64), dim3(64), 0, B.stream, B.slots, B.params, B.out,
+                     B.nslots);
+  return hipGetLastError();
+}
+hipError_t launch_lauum_adjoint(const Batch& B) {
+  dim3 grid(tri(B.nblk_max), B.nslots);
+  const int dmax_batch = B.dmax;
+  if (dmax_batch <= 4)
+    hipLaunchKernelGGL(k_lauum_adjoint<4>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  else if (dmax_batch <= 8)
+    hipLaunchKernelGGL(k_lauum_adjoint<8>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  else if (dmax_batch <= 16)
+    hipLaunchKernelGGL(k_lauum_adjoint<16>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  else
+    hipLaunchKernelGGL(k_lauum_adjoint<32>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  return hipGetLastError();
+}
+hipError_t launch_finalize_grad(const Batch& B) {
+  hipLaunchKernelGGL(k_finalize_grad, dim3(B.nslots), dim3(GRAD_STRIDE), 0, B.stream, B.slots, B.params, B.out,
+                     B.nslots);
+  return hipGetLastError();
+}
+hipError_t launch_rcond_estimate(const Batch& B) {
+  hipLaunchKernelGGL(k_rcond_estimate, dim3(B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  return hipGetLastError();
+}
+
+}  // namespace lk

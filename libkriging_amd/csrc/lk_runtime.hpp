@@ -1,0 +1,139 @@
+// Host runtime of the batched likelihood pipeline: device memory for designs
+// and batch slots, the launch sequence, hipGraph capture, and the
+// jitter-retry ladder of LinearAlgebra::safe_chol_lower_retry
+// (src/lib/LinearAlgebra.cpp:68-98) driven from the per-slot device status.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lk_kernels.h"
+
+namespace lk {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what);
+
+// LinearAlgebra statics (LinearAlgebra.cpp:33,53,63,100)
+struct CholConfig {
+  double num_nugget = 1e-10;
+  bool rcond_check = true;
+  int max_inc = 10;
+  double min_rcond = 1e-18;
+};
+
+// Estimation flags of a model (Kriging::_logLikelihood, Kriging.cpp:214-245)
+struct ModelFlags {
+  bool est_sigma2 = true;
+  bool est_nugget = true;
+  double sigma2 = 1.0;
+  double nugget = 0.0;
+  double alpha = 1.0;  // member alpha used when gamma has d entries
+};
+
+struct EvalRequest {
+  int problem = 0;
+  const double* gamma = nullptr;  // d or d+1 entries
+  int gamma_n = 0;
+  ModelFlags flags;
+};
+
+enum EvalStatus { EVAL_OK = 0, EVAL_CHOL_FAILED = 1, EVAL_NUGGET_NOT_POSITIVE = 2 };
+
+struct EvalResult {
+  double ll = 0.0;
+  double grad[DMAX + 1] = {};
+  double beta[PMAX] = {};
+  double sse = 0.0, logdet = 0.0, sigma2 = 0.0, rcond2 = 0.0;
+  int status = EVAL_OK;
+  int inc = 0;
+};
+
+struct DeviceProblem {
+  int n = 0, d = 0, p = 0, kernel = 0, noise = 0;
+  double* X = nullptr;
+  double* y = nullptr;
+  double* F = nullptr;
+};
+
+// Per-evaluation timings filled when a bench map is requested (Bench::toc
+// semantics, src/lib/Bench.cpp:16-35), milliseconds.
+using BenchMap = std::map<std::string, double>;
+
+class Runtime {
+ public:
+  explicit Runtime(int device);
+  ~Runtime();
+  Runtime(const Runtime&) = delete;
+  Runtime& operator=(const Runtime&) = delete;
+
+  int add_problem(const double* X, int n, int d, const double* y, const double* F, int p, int kernel, int noise);
+  void remove_problem(int id);
+  const DeviceProblem& problem(int id) const;
+
+  // Evaluate LL (and the gradient over gamma) for every request in one
+  // batched pipeline.  Per-request failures (chol still failing after the
+  // retry ladder) are reported in EvalResult::status.
+  void evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std::vector<EvalResult>& res,
+                BenchMap* bench = nullptr);
+
+  // Device pointers of slot s after the last evaluate(): the factor L (A,
+  // lower, ld = npad), L^-1 (W lower) and the vector block V.
+  struct SlotView {
+    const double* A;
+    const double* W;
+    const double* V;
+    int n, npad, p;
+  };
+  SlotView slot_view(int s) const;
+  // host copy of a slot buffer after the last evaluate(); `what` as
+  // LKGPU_READ_* in include/lkgpu.h
+  void read_slot(int s, int what, double* out);
+
+  CholConfig chol;
+  bool use_graph = true;
+  hipStream_t stream() const { return stream_; }
+  int device() const { return device_; }
+
+ private:
+  void ensure_slots(const std::vector<EvalRequest>& reqs);
+  void run_pipeline(bool want_grad, BenchMap* bench);
+  void launch_all(const Batch& B, bool want_grad);
+
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  std::map<int, DeviceProblem> problems_;
+  int next_problem_ = 0;
+
+  // slot workspace
+  std::vector<int> slot_problem_;
+  std::vector<size_t> slot_bytes_;
+  std::vector<Slot> h_slots_;
+  Slot* d_slots_ = nullptr;
+  SlotParams* h_params_ = nullptr;  // pinned
+  SlotParams* d_params_ = nullptr;
+  SlotOut* h_out_ = nullptr;        // pinned
+  SlotOut* d_out_ = nullptr;
+  size_t slot_cap_ = 0;
+  double* pool_ = nullptr;
+  size_t pool_bytes_ = 0;
+  int nslots_ = 0, nblk_max_ = 0, dmax_ = 0;
+  uint64_t layout_version_ = 0;
+
+  // graph cache: (layout_version, want_grad) -> exec
+  struct GraphEntry {
+    uint64_t version = ~0ull;
+    hipGraphExec_t exec = nullptr;
+  };
+  GraphEntry graphs_[2];
+};
+
+}  // namespace lk

@@ -1,0 +1,205 @@
+"""ctypes front-end of the CPU oracle (oracle/lk_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker and by bench.py's cpu_baseline leg.  The product package
+(libkriging_amd) never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblk_oracle.so")
+
+KERNELS = {"gauss": 0, "exp": 1, "matern3_2": 2, "matern5_2": 3}
+NOISE_NONE, NOISE_NUGGET = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class CholCfg(ctypes.Structure):
+    _fields_ = [("num_nugget", ctypes.c_double), ("rcond_check", ctypes.c_int),
+                ("max_inc", ctypes.c_int), ("min_rcond", ctypes.c_double)]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("d", ctypes.c_int), ("p", ctypes.c_int),
+                ("X", _dp), ("y", _dp), ("F", _dp),
+                ("kernel", ctypes.c_int), ("noise", ctypes.c_int),
+                ("est_sigma2", ctypes.c_int), ("est_nugget", ctypes.c_int),
+                ("est_beta", ctypes.c_int),
+                ("sigma2", ctypes.c_double), ("nugget", ctypes.c_double),
+                ("alpha", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle not built: run `make -C {_HERE}`")
+        L = ctypes.CDLL(_LIB_PATH)
+        L.lko_cov.restype = ctypes.c_double
+        L.lko_cov.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
+        L.lko_dlncov_dtheta.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
+        L.lko_build_R.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                  ctypes.c_double, _dp, _dp]
+        L.lko_potrf_lower.restype = ctypes.c_int
+        L.lko_potrf_lower.argtypes = [_dp, ctypes.c_int]
+        L.lko_trcon_lower.restype = ctypes.c_double
+        L.lko_trcon_lower.argtypes = [_dp, ctypes.c_int]
+        L.lko_safe_chol_lower.restype = ctypes.c_int
+        L.lko_safe_chol_lower.argtypes = [_dp, ctypes.c_int, ctypes.POINTER(CholCfg), _dp, _ip, _dp]
+        L.lko_inv_sympd.argtypes = [_dp, ctypes.c_int, _dp]
+        L.lko_log_likelihood.restype = ctypes.c_int
+        L.lko_log_likelihood.argtypes = [ctypes.POINTER(Problem), _dp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(CholCfg), _dp, _dp, _ip]
+        L.lko_theta_bounds.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.lko_predict.restype = ctypes.c_int
+        L.lko_predict.argtypes = [ctypes.POINTER(Problem), _dp, ctypes.c_double, ctypes.c_double,
+                                  ctypes.POINTER(CholCfg), _dp, ctypes.c_int, _dp, _dp]
+        L.lko_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def default_cfg(num_nugget=1e-10, rcond_check=True, max_inc=10, min_rcond=1e-18):
+    return CholCfg(num_nugget, int(rcond_check), max_inc, min_rcond)
+
+
+def trend_matrix(X, regmodel):
+    """Trend::regressionModelMatrix (Trend.cpp:34-61) for constant / linear."""
+    X = np.asarray(X, dtype=np.float64)
+    n = X.shape[0]
+    if regmodel == "constant":
+        return np.ones((n, 1))
+    if regmodel == "linear":
+        return np.hstack([np.ones((n, 1)), X])
+    if regmodel == "none":
+        return np.ones((n, 0))
+    raise ValueError(regmodel)
+
+
+class OracleProblem:
+    """Keeps the arrays alive while the C struct points at them."""
+
+    def __init__(self, X, y, kernel="gauss", regmodel="constant", noise=None,
+                 est_sigma2=True, sigma2=1.0, est_nugget=True, nugget=0.0, alpha=1.0,
+                 est_beta=True):
+        X = np.asarray(X, dtype=np.float64)
+        if X.ndim == 1:
+            X = X.reshape(-1, 1)
+        self.X = _f64(X)
+        self.y = _f64(np.asarray(y, dtype=np.float64).reshape(-1))
+        self.F = _f64(trend_matrix(self.X, regmodel))
+        n, d = self.X.shape
+        self.n, self.d, self.p = n, d, self.F.shape[1]
+        self.kernel = kernel
+        self.noise = noise
+        self.s = Problem(n, d, self.p, _ptr(self.X), _ptr(self.y), _ptr(self.F),
+                         KERNELS[kernel], NOISE_NUGGET if noise == "nugget" else NOISE_NONE,
+                         int(est_sigma2), int(est_nugget), int(est_beta),
+                         float(sigma2), float(nugget), float(alpha))
+
+    def log_likelihood(self, gamma, want_grad=True, cfg=None):
+        gamma = _f64(np.asarray(gamma, dtype=np.float64).reshape(-1))
+        cfg = cfg or default_cfg()
+        ll = ctypes.c_double()
+        grad = np.zeros(gamma.size)
+        inc = ctypes.c_int(0)
+        rc = lib().lko_log_likelihood(ctypes.byref(self.s), _ptr(gamma), gamma.size, int(want_grad),
+                                      ctypes.byref(cfg), ctypes.byref(ll), _ptr(grad), ctypes.byref(inc))
+        if rc != 0:
+            raise RuntimeError(lib().lko_last_error().decode())
+        return ll.value, grad, inc.value
+
+    def build_R(self, theta, factor=1.0):
+        theta = _f64(theta)
+        R = np.zeros((self.n, self.n), order="F")
+        lib().lko_build_R(KERNELS[self.kernel], _ptr(self.X), self.n, self.d, _ptr(theta),
+                          float(factor), None, _ptr(R))
+        return R
+
+    def predict(self, theta, alpha, sigma2, Xn, cfg=None):
+        theta = _f64(theta)
+        Xn = _f64(np.asarray(Xn, dtype=np.float64).reshape(-1, self.d))
+        m = Xn.shape[0]
+        mean = np.zeros(m)
+        sd = np.zeros(m)
+        cfg = cfg or default_cfg()
+        rc = lib().lko_predict(ctypes.byref(self.s), _ptr(theta), float(alpha), float(sigma2),
+                               ctypes.byref(cfg), _ptr(Xn), m, _ptr(mean), _ptr(sd))
+        if rc != 0:
+            raise RuntimeError(lib().lko_last_error().decode())
+        return mean, sd
+
+
+def cov(kernel, dx, theta):
+    dx = _f64(dx)
+    theta = _f64(theta)
+    return lib().lko_cov(KERNELS[kernel], _ptr(dx), _ptr(theta), dx.size)
+
+
+def dlncov_dtheta(kernel, dx, theta):
+    dx = _f64(dx)
+    theta = _f64(theta)
+    out = np.zeros(dx.size)
+    lib().lko_dlncov_dtheta(KERNELS[kernel], _ptr(dx), _ptr(theta), dx.size, _ptr(out))
+    return out
+
+
+def potrf_lower(A):
+    L = _f64(np.array(A, dtype=np.float64))
+    info = lib().lko_potrf_lower(_ptr(L), L.shape[0])
+    return L, info
+
+
+def trcon_lower(L):
+    L = _f64(L)
+    return lib().lko_trcon_lower(_ptr(L), L.shape[0])
+
+
+def safe_chol_lower(R, cfg=None):
+    R = _f64(R)
+    n = R.shape[0]
+    L = np.zeros((n, n), order="F")
+    inc = ctypes.c_int(0)
+    rc_ = ctypes.c_double(0)
+    cfg = cfg or default_cfg()
+    rc = lib().lko_safe_chol_lower(_ptr(R), n, ctypes.byref(cfg), _ptr(L), ctypes.byref(inc),
+                                   ctypes.byref(rc_))
+    if rc != 0:
+        raise RuntimeError(lib().lko_last_error().decode())
+    return L, inc.value, rc_.value
+
+
+def inv_sympd(L):
+    L = _f64(L)
+    n = L.shape[0]
+    out = np.zeros((n, n), order="F")
+    lib().lko_inv_sympd(_ptr(L), n, _ptr(out))
+    return out
+
+
+def theta_bounds(X, y, lower_factor=0.02, upper_factor=10.0, variogram=True):
+    X = _f64(X)
+    y = _f64(np.asarray(y).reshape(-1))
+    n, d = X.shape
+    lo = np.zeros(d)
+    up = np.zeros(d)
+    lib().lko_theta_bounds(_ptr(X), _ptr(y), n, d, lower_factor, upper_factor, int(variogram),
+                           _ptr(lo), _ptr(up))
+    return lo, up
