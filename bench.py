@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Headline benchmark: batched FP64 LL + adjoint-gradient evaluations per
+second on one MI355X, against the plain one-evaluation-per-launch GPU path.
+
+Workload (the 64-start multistart fit configuration): n = 1024 design points
+X ~ U[0,1)^6, y = prod_j (1 + 0.5 cos(3 x_j)) + N(0, 1e-4), exp kernel,
+constant trend, 64 parameter vectors theta ~ log-U[0.05, 2]^6 -- all drawn
+with numpy's PCG64 from a fixed seed.  One step = one batched evaluation of
+the 64 LL + gradients (Cholesky, L^-1, GLS solves, R^-1 and the adjoint
+gradient, all FP64) with X and y resident in HBM.
+
+Prints ONE JSON line (rank 0, N = 1).  Extra fields: plain_evals_per_s (the
+same 64 evaluations as 64 single-item launches) and batched_vs_plain.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_POINTS, DIM, NSTART = 1024, 6, 64
+SEED = 126  # 123 + index of the multistart configuration
+MFMA_FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 256 << 20
+
+
+def workload(seed=SEED, n=N_POINTS, d=DIM, nstart=NSTART):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(size=(n, d))
+    y = np.prod(1.0 + 0.5 * np.cos(3.0 * X), axis=1) + rng.normal(0.0, 1e-2, size=n)
+    theta = np.exp(rng.uniform(np.log(0.05), np.log(2.0), size=(nstart, d)))
+    return X, y, theta
+
+
+def cpu_baseline(X, y, theta, gpu_ll, gpu_grad, budget_s=15.0, max_evals=20):
+    """The CPU oracle (a C restatement of the reference algorithm, 1 thread)
+    timed on a bounded sample of the same 64 evaluations; also returns the
+    parity of the GPU results on that sample."""
+    from oracle import oracle as O
+    P = O.OracleProblem(X, y, "exp", "constant")
+    t0 = time.perf_counter()
+    k = 0
+    ll_err, g_err = 0.0, 0.0
+    while k < max_evals and (time.perf_counter() - t0) < budget_s:
+        ll, g, _ = P.log_likelihood(theta[k])
+        ll_err = max(ll_err, abs(gpu_ll[k] - ll) / abs(ll))
+        g_err = max(g_err, np.linalg.norm(gpu_grad[k] - g) / np.linalg.norm(g))
+        k += 1
+    dt = time.perf_counter() - t0
+    return {"value": k / dt, "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": f"{k} of the {theta.shape[0]} LL+grad evaluations (n={X.shape[0]}, d={X.shape[1]}), "
+                      f"single-threaded C oracle"}, ll_err, g_err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stage", default="adjoint", help="kernel timed live for the roofline")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help=f"also run a {COPY_BYTES >> 20} MiB device copy (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the multistart batch would shard by starts, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from libkriging_amd import _native as N
+
+    X, y, theta = workload()
+    n, d = X.shape
+    ctx = N.Context(0)
+    pid = ctx.problem(X, y, np.ones((n, 1)), "exp")
+    ids = [pid] * NSTART
+
+    # batched path: one hipGraph replay per step
+    for _ in range(args.warmup):
+        ctx.ll_grad_batch(ids, theta, True)
+    ctx.set_timed_stage(args.stage)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ll, grad, st, inc = ctx.ll_grad_batch(ids, theta, True)
+    ctx.synchronize()
+    t_batched = time.perf_counter() - t0
+    stage_ms, stage_count = ctx.take_stage_time()
+    ctx.set_timed_stage(-1)
+    assert (st == 0).all(), "an evaluation failed"
+    evals_s = NSTART * args.steps / t_batched
+
+    # plain path: the same evaluations, one per launch sequence
+    for b in range(2):
+        ctx.ll_grad_batch([pid], theta[b:b + 1], True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        for b in range(NSTART):
+            l1, g1, _, _ = ctx.ll_grad_batch([pid], theta[b:b + 1], True)
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t0
+    plain_evals_s = NSTART * args.plain_steps / t_plain if args.plain_steps > 0 else float("nan")
+    # same numbers from both paths
+    l1, g1, _, _ = ctx.ll_grad_batch([pid], theta[NSTART - 1:NSTART], True)
+    same = bool(l1[0] == ll[NSTART - 1] and np.array_equal(g1[0], grad[NSTART - 1]))
+
+    # roofline of the live-timed kernel
+    if args.stage == "adjoint":
+        # R^-1 = L^-T L^-1 on MFMA: n^3/3 flops per evaluation
+        flops = NSTART * n ** 3 / 3.0
+        achieved = flops / (stage_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / MFMA_FP64_PEAK_TFLOPS, "traffic": None,
+                "kernel": "k_lauum_adjoint", "avg_ms": stage_ms, "launches": stage_count}
+    elif args.stage == "build":
+        nbytes = NSTART * (8.0 * n * (n + 1) / 2 + 8.0 * n * d)
+        achieved = nbytes / (stage_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_build",
+                "avg_ms": stage_ms, "launches": stage_count}
+    else:
+        roof = {"bound": "mfma", "achieved": None, "peak": MFMA_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": None, "traffic": None, "stage": args.stage, "avg_ms": stage_ms}
+
+    line = {
+        "metric": "LL+grad evals/s (batched adjoint vs plain GPU), n=1024 d=6, rel err<=1e-8",
+        "value": evals_s,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_batched / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "64-start batched LL+grad, n=1024 d=6 exp kernel, constant trend",
+                   "n": n, "d": d, "kernel": "exp", "batch": NSTART, "seed": SEED},
+        "plain_evals_per_s": plain_evals_s,
+        "batched_vs_plain": evals_s / plain_evals_s,
+        "plain_equals_batched": same,
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        cb, ll_err, g_err = cpu_baseline(X, y, theta, ll, grad)
+        line["cpu_baseline"] = cb
+        line["parity_vs_oracle"] = {"ll_rel_max": ll_err, "grad_rel_max": g_err}
+    print(json.dumps(line))
+    if args.copy_calibration:
+        # a plain device copy of known size, run in the same process so that
+        # a counter-collection pass over this command also measures it:
+        # FETCH_SIZE / WRITE_SIZE of this kernel calibrate those counters
+        import torch
+        src = torch.empty(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": COPY_BYTES}), file=sys.stderr)
+
+
+if __name__ == "__main__":
+    main()

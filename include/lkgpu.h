@@ -99,8 +99,119 @@ int lkgpu_ll_grad_batch(lkgpu_context* ctx, int nb, const int* problem_ids, cons
 #define LKGPU_READ_X 5     /* n       L^-T E* = R^-1 (y - F beta) */
 int lkgpu_read_item(lkgpu_context* ctx, int item, int what, double* out);
 
+/* Live timing of one pipeline stage: a HIP event pair is recorded on the
+ * pipeline stream around the stage (inside the replayed graph), and every
+ * batch call adds its elapsed time.  lkgpu_take_stage_time returns the mean
+ * per batch call since the last take (and resets).  No reference
+ * counterpart: the reference's Bench::tic/toc (src/lib/Bench.cpp:16-35)
+ * times host steps. */
+#define LKGPU_STAGE_NONE -1
+#define LKGPU_STAGE_BUILD 0         /* covariance build (one launch) */
+#define LKGPU_STAGE_CHOLESKY 1      /* all Cholesky launches */
+#define LKGPU_STAGE_UPDATE0 2       /* first trailing update (one launch) */
+#define LKGPU_STAGE_TRTRI 3         /* all L^-1 launches */
+#define LKGPU_STAGE_SOLVES 4        /* GLS solves, LL, condition check */
+#define LKGPU_STAGE_ADJOINT 5       /* fused R^-1 + adjoint gradient (one launch) */
+int lkgpu_set_timed_stage(lkgpu_context* ctx, int stage);
+int lkgpu_take_stage_time(lkgpu_context* ctx, double* avg_ms, int* count);
+
+/* Wait for all work on the context's device (hipDeviceSynchronize). */
+int lkgpu_synchronize(lkgpu_context* ctx);
+
 /* Use (1) or bypass (0) hipGraph replay of the evaluation pipeline. */
 int lkgpu_set_graph(lkgpu_context* ctx, int enable);
+
+/* ------------------------------------------------------------------------ */
+/* Layer 2: the Kriging object (libkriging_c.h names and conventions)       */
+/* Supported here: kernels gauss/exp/matern3_2/matern5_2, noise "none" or   */
+/* "nugget", regmodel none/constant/linear, objective "LL", optim "none" or */
+/* "BFGS[#]".  Other values fail with -1 / NULL and a message.              */
+/* ------------------------------------------------------------------------ */
+
+/* libkriging_c.h:19 */
+void* lk_kriging_new(const char* kernel, const char* noise_model);
+/* libkriging_c.h:20-43.  theta holds theta_n = rows x d values (row-major);
+ * rows > 1 gives several BFGS starting points (the reference C ABI passes one
+ * row). noise / noise_n must be NULL / 0 (no heterogeneous noise). */
+void* lk_kriging_new_fit(const double* y, int n, const double* noise, int noise_n, const double* X, int nX, int d,
+                         const char* kernel, const char* noise_model, const char* regmodel, int normalize,
+                         const char* optim, const char* objective, const double* sigma2, int is_sigma2_estim,
+                         const double* theta, int theta_n, int is_theta_estim, const double* beta, int beta_n,
+                         int is_beta_estim, const double* nugget, int is_nugget_estim);
+/* libkriging_c.h:44 */
+void lk_kriging_delete(void* ptr);
+/* libkriging_c.h:47-58 */
+int lk_kriging_fit(void* ptr, const double* y, int n, const double* noise, int noise_n, const double* X, int nX, int d,
+                   const char* regmodel, int normalize, const char* optim, const char* objective);
+/* fit with starting / fixed parameters, as the Python binding's
+ * fit(..., parameters=dict) (Kriging_binding.cpp:39-48); no counterpart in
+ * libkriging_c.h, which only offers them through lk_kriging_new_fit */
+int lk_kriging_fit_parameters(void* ptr, const double* y, int n, const double* X, int nX, int d, const char* regmodel,
+                              int normalize, const char* optim, const char* objective, const double* sigma2,
+                              int is_sigma2_estim, const double* theta, int theta_n, int is_theta_estim,
+                              const double* beta, int beta_n, int is_beta_estim, const double* nugget,
+                              int is_nugget_estim);
+/* libkriging_c.h:60-71; return_cov / return_deriv are rejected */
+int lk_kriging_predict(void* ptr, const double* X_n, int m, int d, int return_stdev, int return_cov, int return_deriv,
+                       double* mean_out, double* stdev_out, double* cov_out, double* mean_deriv_out,
+                       double* stdev_deriv_out);
+/* libkriging_c.h:109-116 (Kriging::logLikelihoodFun, Kriging.cpp:343-349);
+ * return_hess is rejected */
+int lk_kriging_log_likelihood_fun(void* ptr, const double* theta, int theta_n, int return_grad, int return_hess,
+                                  double* ll_out, double* grad_out, double* hess_out);
+/* nb parameter vectors (rows of theta, each theta_n long) in one batched
+ * pipeline; no reference counterpart (the reference evaluates one at a time) */
+int lk_kriging_log_likelihood_fun_batch(void* ptr, int nb, const double* theta, int theta_n, int return_grad,
+                                        double* ll_out, double* grad_out);
+/* libkriging_c.h:131 */
+double lk_kriging_log_likelihood(void* ptr);
+
+/* getters, libkriging_c.h:141-166 (normalised-space members) */
+const char* lk_kriging_kernel(void* ptr);
+const char* lk_kriging_optim(void* ptr);
+const char* lk_kriging_objective(void* ptr);
+int lk_kriging_is_normalize(void* ptr);
+const char* lk_kriging_regmodel(void* ptr);
+const char* lk_kriging_noise_model(void* ptr);
+int lk_kriging_get_X(void* ptr, double* out, int* n, int* d);
+int lk_kriging_get_centerX(void* ptr, double* out, int* d);
+int lk_kriging_get_scaleX(void* ptr, double* out, int* d);
+int lk_kriging_get_y(void* ptr, double* out, int* n);
+double lk_kriging_get_centerY(void* ptr);
+double lk_kriging_get_scaleY(void* ptr);
+int lk_kriging_get_beta(void* ptr, double* out, int* n);
+int lk_kriging_get_theta(void* ptr, double* out, int* n);
+double lk_kriging_get_sigma2(void* ptr);
+double lk_kriging_get_nugget(void* ptr);
+
+/* evaluations and batched launches of the last fit; no reference counterpart */
+int lk_kriging_fit_statistics(void* ptr, int* evaluations, int* batches);
+/* 1: the fit evaluates all pending starts in one batched launch (default);
+ * 0: one launch sequence per evaluation (for comparison); no reference
+ * counterpart */
+int lk_kriging_set_batched_fit(void* ptr, int batched);
+
+/* Process-global settings.  lk_optim_set/get cover the Optim statics
+ * (src/lib/Optim.cpp:39-141: reparametrize, theta_lower_factor,
+ * theta_upper_factor, variogram_bounds_heuristic, log_level, max_restart,
+ * max_iteration, gradient_tolerance, objective_rel_tolerance) plus
+ * lbfgs_memory; lk_linalg_set covers the LinearAlgebra statics of the
+ * default device (num_nugget, chol_rcond_check, max_inc_choldiag, min_rcond;
+ * LinearAlgebra.cpp:33-100); lk_random_reset_seed is Random::reset_seed
+ * (Random.cpp:17-20). */
+int lk_optim_set(const char* name, double value);
+int lk_optim_get(const char* name, double* value);
+int lk_linalg_set(const char* name, double value);
+int lk_random_reset_seed(unsigned seed);
+
+/* The bound-constrained optimiser the fit uses, on a user objective
+ * (replaces lbfgsb::Optimizer::minimize, src/lib/Kriging.cpp:1980-1987).
+ * fn returns f(x) and writes g(x).  x is overwritten with the solution;
+ * task_out receives the termination string. */
+typedef double (*lk_objective_fn)(const double* x, double* g, void* user);
+int lk_lbfgsb_minimize(int n, double* x, const double* lower, const double* upper, lk_objective_fn fn, void* user,
+                       int max_iter, double pgtol, double factr, int m, double* f_out, int* iters_out,
+                       int* evals_out, char* task_out, int task_len);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,12 @@ SIGNATURES = {
     "lkgpu_ll_grad_batch": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, ctypes.c_int, _vp, ctypes.c_int,
                                            ctypes.c_int, _dp, _dp, _ip, _ip]),
     "lkgpu_read_item": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "lkgpu_synchronize": (ctypes.c_int, [_vp]),
+    "lkgpu_set_timed_stage": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lkgpu_take_stage_time": (ctypes.c_int, [_vp, _dp, _ip]),
 }
+
+STAGES = {"build": 0, "cholesky": 1, "update0": 2, "trtri": 3, "solves": 4, "adjoint": 5}
 
 READ_L, READ_LINV, READ_FSTAR, READ_YSTAR, READ_ESTAR, READ_X = range(6)
 
@@ -150,6 +155,18 @@ class Context:
             ll.ctypes.data_as(_dp), grad.ctypes.data_as(_dp) if return_grad else None,
             st.ctypes.data_as(_ip), inc.ctypes.data_as(_ip)))
         return ll, grad, st, inc
+
+    def synchronize(self):
+        check(lib().lkgpu_synchronize(self.h))
+
+    def set_timed_stage(self, stage):
+        check(lib().lkgpu_set_timed_stage(self.h, STAGES[stage] if isinstance(stage, str) else int(stage)))
+
+    def take_stage_time(self):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int()
+        check(lib().lkgpu_take_stage_time(self.h, ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
 
     def read_item(self, item, what):
         """Host copy of a factor of batch item `item` of the last batch."""

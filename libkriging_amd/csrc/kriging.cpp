@@ -1,0 +1,753 @@
+#include "kriging.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+
+#include "lbfgsb.hpp"
+
+namespace lk {
+
+// ---------------------------------------------------------------------------
+// process-global state: optimiser knobs (Optim.cpp:16-141, with the LK_* env
+// overrides of get_env_or_default), the RNG (Random.cpp) and one runtime per
+// device
+// ---------------------------------------------------------------------------
+namespace {
+template <class T>
+T env_or(const char* name, T def) {
+  const char* v = std::getenv(name);
+  if (v == nullptr) return def;
+  try {
+    if constexpr (std::is_same<T, bool>::value) {
+      std::string s(v);
+      return s == "1" || s == "true" || s == "TRUE" || s == "True";
+    } else if constexpr (std::is_integral<T>::value) {
+      return (T)std::stol(v);
+    } else {
+      return (T)std::stod(v);
+    }
+  } catch (...) {
+    return def;
+  }
+}
+
+OptimConfig make_optim_config() {
+  OptimConfig c;
+  c.reparametrize = env_or("LK_REPARAMETRIZE", true);
+  c.theta_lower_factor = env_or("LK_THETA_LOWER_FACTOR", 0.02);
+  c.theta_upper_factor = env_or("LK_THETA_UPPER_FACTOR", 10.0);
+  c.variogram_bounds_heuristic = env_or("LK_VARIOGRAM_BOUNDS_HEURISTIC", true);
+  c.log_level = env_or("LK_LOG_LEVEL", 0);
+  c.max_restart = env_or("LK_MAX_RESTART", 10);
+  c.max_iteration = env_or("LK_MAX_ITERATION", 20);
+  c.gradient_tolerance = env_or("LK_GRADIENT_TOLERANCE", 1e-3);
+  c.objective_rel_tolerance = env_or("LK_OBJECTIVE_REL_TOLERANCE", 1e-3);
+  return c;
+}
+
+unsigned g_seed = 123;
+std::mt19937 g_engine(123);
+std::mutex g_rt_mutex;
+std::map<int, std::unique_ptr<Runtime>> g_runtimes;
+
+constexpr double NUGGET_ALPHA_LOWER = 1e-3;  // Kriging.cpp:678
+}  // namespace
+
+OptimConfig& optim_config() {
+  static OptimConfig c = make_optim_config();
+  return c;
+}
+
+void random_reset_seed(unsigned seed) {
+  g_seed = seed;
+  g_engine.seed(seed);
+}
+
+int default_device() { return env_or("LKGPU_DEVICE", 0); }
+
+Runtime& runtime_for(int device) {
+  std::lock_guard<std::mutex> lock(g_rt_mutex);
+  auto it = g_runtimes.find(device);
+  if (it == g_runtimes.end()) it = g_runtimes.emplace(device, std::make_unique<Runtime>(device)).first;
+  return *it->second;
+}
+
+RegModel regmodel_from_string(const std::string& s) {
+  if (s == "none") return RegModel::None;
+  if (s == "constant") return RegModel::Constant;
+  if (s == "linear") return RegModel::Linear;
+  throw std::invalid_argument("Unsupported regmodel on this path: " + s + " (constant, linear, none)");
+}
+
+int kernel_id(const std::string& name) {
+  if (name == "gauss") return K_GAUSS;
+  if (name == "exp") return K_EXP;
+  if (name == "matern3_2") return K_MATERN32;
+  if (name == "matern5_2") return K_MATERN52;
+  throw std::invalid_argument("Unsupported covariance kernel: " + name);
+}
+
+// ---------------------------------------------------------------------------
+Kriging::Kriging(const std::string& kernel, NoiseModel noise) : kernel_(kernel), noise_(noise) {
+  kernel_id(kernel);  // validates the name (Covariance::resolve)
+  device_ = default_device();
+}
+
+Kriging::~Kriging() {
+  if (rt_ != nullptr) {
+    hipSetDevice(device_);
+    if (problem_ >= 0) {
+      try {
+        rt_->remove_problem(problem_);
+      } catch (...) {
+      }
+    }
+    if (d_Linv_) hipFree(d_Linv_);
+    if (d_vec_) hipFree(d_vec_);
+  }
+}
+
+// fit_setup_impl (KrigingImpl.cpp:764-841): normalisation, trend matrix
+void Kriging::setup(const double* y, const double* X, int n, int d, RegModel regmodel, bool normalize) {
+  if (n <= 0 || d <= 0) throw std::invalid_argument("empty design");
+  n_ = n;
+  d_ = d;
+  regmodel_ = regmodel;
+  normalize_ = normalize;
+  centerX_.assign(d, 0.0);
+  scaleX_.assign(d, 1.0);
+  centerY_ = 0.0;
+  scaleY_ = 1.0;
+  if (normalize) {
+    for (int k = 0; k < d; ++k) {
+      double mn = X[(size_t)k * n], mx = mn;
+      for (int i = 1; i < n; ++i) {
+        mn = std::min(mn, X[i + (size_t)k * n]);
+        mx = std::max(mx, X[i + (size_t)k * n]);
+      }
+      centerX_[k] = mn;
+      scaleX_[k] = mx - mn;
+    }
+    double mn = y[0], mx = y[0];
+    for (int i = 1; i < n; ++i) {
+      mn = std::min(mn, y[i]);
+      mx = std::max(mx, y[i]);
+    }
+    centerY_ = mn;
+    scaleY_ = mx - mn;
+  }
+  X_.resize((size_t)n * d);
+  y_.resize(n);
+  for (int k = 0; k < d; ++k)
+    for (int i = 0; i < n; ++i) X_[i + (size_t)k * n] = (X[i + (size_t)k * n] - centerX_[k]) / scaleX_[k];
+  for (int i = 0; i < n; ++i) y_[i] = (y[i] - centerY_) / scaleY_;
+  // Trend::regressionModelMatrix (Trend.cpp:34-61)
+  p_ = regmodel == RegModel::None ? 0 : (regmodel == RegModel::Constant ? 1 : 1 + d);
+  F_.assign((size_t)n * std::max(p_, 1), 0.0);
+  if (p_ > 0)
+    for (int i = 0; i < n; ++i) F_[i] = 1.0;
+  if (regmodel == RegModel::Linear)
+    for (int k = 0; k < d; ++k)
+      for (int i = 0; i < n; ++i) F_[i + (size_t)(k + 1) * n] = X_[i + (size_t)k * n];
+
+  rt_ = &runtime_for(device_);
+  if (problem_ >= 0) rt_->remove_problem(problem_);
+  problem_ = rt_->add_problem(X_.data(), n, d, y_.data(), F_.data(), p_, kernel_id(kernel_), (int)noise_);
+  is_empty_ = true;
+}
+
+ModelFlags Kriging::flags_for_eval() const {
+  ModelFlags f;
+  f.est_sigma2 = est_sigma2_;
+  f.est_nugget = est_nugget_;
+  f.sigma2 = sigma2_;
+  f.nugget = nugget_;
+  f.alpha = alpha_;
+  return f;
+}
+
+// Optim::theta_bounds (Optim.cpp:179-209)
+void Kriging::theta_bounds(std::vector<double>& lo, std::vector<double>& up) const {
+  const OptimConfig& oc = optim_config();
+  const int n = n_, d = d_;
+  lo.assign(d, 0.0);
+  up.assign(d, 0.0);
+  for (int k = 0; k < d; ++k) {
+    double mn = X_[(size_t)k * n], mx = mn;
+    for (int i = 1; i < n; ++i) {
+      mn = std::min(mn, X_[i + (size_t)k * n]);
+      mx = std::max(mx, X_[i + (size_t)k * n]);
+    }
+    lo[k] = oc.theta_lower_factor * (mx - mn);
+    up[k] = oc.theta_upper_factor * (mx - mn);
+  }
+  if (!oc.variogram_bounds_heuristic || n <= 0) return;
+  // slope weights w_ij = (y_i - y_j)^2 / |x_i - x_j|^2 over ordered pairs
+  // (NaN -> 0), normalised; lower bound raised to factor * sum w |dx|
+  double wsum = 0.0;
+  std::vector<double> acc(d, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < d; ++k) {
+        const double v = X_[i + (size_t)k * n] - X_[j + (size_t)k * n];
+        s += v * v;
+      }
+      const double dy = y_[i] - y_[j];
+      double w = (dy * dy) / s;
+      if (std::isnan(w)) w = 0.0;
+      wsum += 2.0 * w;
+      for (int k = 0; k < d; ++k) acc[k] += 2.0 * w * std::fabs(X_[i + (size_t)k * n] - X_[j + (size_t)k * n]);
+    }
+  if (wsum > 0.0) {
+    for (int k = 0; k < d; ++k) {
+      lo[k] = std::max(lo[k], oc.theta_lower_factor * (acc[k] / wsum));
+      lo[k] = std::min(lo[k], up[k]);
+      up[k] = std::max(lo[k], up[k]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// likelihood API
+// ---------------------------------------------------------------------------
+double Kriging::logLikelihoodFun(const double* gamma, int gamma_n, bool want_grad, double* grad) {
+  double ll = 0.0;
+  logLikelihoodFunBatch(1, gamma, gamma_n, want_grad, &ll, grad);
+  return ll;
+}
+
+void Kriging::logLikelihoodFunBatch(int nb, const double* gamma, int gamma_n, bool want_grad, double* ll,
+                                    double* grad) {
+  if (rt_ == nullptr) throw std::runtime_error("model is not fitted (no data)");
+  if (gamma_n != d_ && !(noise_ == NoiseModel::Nugget && gamma_n == d_ + 1))
+    throw std::invalid_argument("theta must have length " + std::to_string(d_) +
+                                (noise_ == NoiseModel::Nugget ? " or " + std::to_string(d_ + 1) : std::string()));
+  std::vector<EvalRequest> reqs(nb);
+  for (int i = 0; i < nb; ++i) {
+    reqs[i].problem = problem_;
+    reqs[i].gamma = gamma + (size_t)i * gamma_n;
+    reqs[i].gamma_n = gamma_n;
+    reqs[i].flags = flags_for_eval();
+  }
+  std::vector<EvalResult> res;
+  rt_->evaluate(reqs, want_grad, res);
+  for (int i = 0; i < nb; ++i) {
+    if (res[i].status != EVAL_OK)
+      throw std::runtime_error("[ERROR] Exceed max numerical nugget (" + std::to_string(res[i].inc) +
+                               " x 1e-10) added to force chol matrix");
+    ll[i] = res[i].ll;
+    if (want_grad)
+      for (int k = 0; k < gamma_n; ++k) grad[(size_t)i * gamma_n + k] = res[i].grad[k];
+  }
+}
+
+double Kriging::logLikelihood() {
+  if (is_empty_) throw std::runtime_error("model is not fitted");
+  std::vector<double> g(theta_);
+  if (noise_ == NoiseModel::Nugget) g.push_back(alpha_);
+  return logLikelihoodFun(g.data(), (int)g.size(), false, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// fit
+// ---------------------------------------------------------------------------
+void Kriging::fit(const double* y, const double* X, int n, int d, RegModel regmodel, bool normalize,
+                  const std::string& optim, const std::string& objective, const Parameters& P) {
+  if (objective != "LL")
+    throw std::invalid_argument("Unsupported objective on this path: " + objective + " (LL)");
+  optim_ = optim;
+  objective_ = objective;
+  setup(y, X, n, d, regmodel, normalize);
+
+  // beta fixed or estimated (KrigingImpl.cpp:816-826)
+  est_beta_ = P.is_beta_estim && regmodel != RegModel::None;
+  if (!est_beta_ && P.beta.has_value() && !P.beta->empty()) {
+    est_beta_ = false;
+    beta_ = *P.beta;
+    if ((int)beta_.size() != p_) throw std::invalid_argument("beta must have length " + std::to_string(p_));
+    if (normalize_)
+      for (double& b : beta_) b /= scaleY_;
+  } else {
+    est_beta_ = true;
+  }
+  // theta0 rows (normalised)
+  std::vector<double> theta0;
+  int rows0 = 0;
+  if (P.theta.has_value()) {
+    theta0 = *P.theta;
+    rows0 = P.theta_rows;
+    if (rows0 <= 0 || (int)theta0.size() != rows0 * d)
+      throw std::runtime_error("Dimension of theta should be nx" + std::to_string(d));
+    if (normalize_)
+      for (int r = 0; r < rows0; ++r)
+        for (int k = 0; k < d; ++k) theta0[(size_t)r * d + k] /= scaleX_[k];
+  }
+
+  if (optim == "none") {
+    if (!P.theta.has_value())
+      throw std::runtime_error("Theta should be given (1x" + std::to_string(d) + ") matrix, when optim=none");
+    std::vector<double> th(theta0.begin(), theta0.begin() + d);
+    est_theta_ = false;
+    double sigma2 = -1;
+    est_sigma2_ = P.is_sigma2_estim;
+    if (P.sigma2.has_value()) {
+      sigma2 = *P.sigma2;
+      if (normalize_) sigma2 /= scaleY_ * scaleY_;
+    } else {
+      est_sigma2_ = true;
+    }
+    double alpha = 1.0, nugget_param = 0.0;
+    if (noise_ == NoiseModel::Nugget) {
+      est_nugget_ = P.is_nugget_estim;
+      if (P.nugget.has_value()) {
+        nugget_param = *P.nugget;
+        if (normalize_) nugget_param /= scaleY_ * scaleY_;
+      }
+      alpha = (sigma2 > 0 && (sigma2 + nugget_param) > 0) ? sigma2 / (sigma2 + nugget_param)
+                                                           : 1.0 - NUGGET_ALPHA_LOWER;
+      sigma2_ = sigma2;
+      nugget_ = nugget_param;
+    } else if (!est_sigma2_) {
+      sigma2_ = sigma2;
+    }
+    alpha_ = alpha;
+    commit(th, alpha);
+    return;
+  }
+  if (optim.rfind("BFGS", 0) != 0)
+    throw std::runtime_error("Unsupported optim: " + optim + " (supported are: none, BFGS[#])");
+  // Optim::parse_method("BFGS64") -> 64 starts
+  int multistart = 1;
+  {
+    const std::string rest = optim.substr(4);
+    size_t k = 0;
+    while (k < rest.size() && std::isdigit((unsigned char)rest[k])) ++k;
+    if (k > 0) multistart = std::max(1, std::atoi(rest.substr(0, k).c_str()));
+  }
+  Parameters Pn = P;
+  if (P.theta.has_value()) {
+    Pn.theta = theta0;
+    Pn.theta_rows = rows0;
+  }
+  fit_bfgs(multistart, Pn);
+}
+
+namespace {
+// gamma <-> (theta, alpha) reparametrisations (Optim.cpp:49-67, Kriging.cpp:674-702)
+struct Reparam {
+  int d;
+  bool nugget;
+  bool log;
+  void to_gamma(const double* theta, double alpha, double* g) const {
+    for (int k = 0; k < d; ++k) g[k] = log ? std::log(theta[k]) : theta[k];
+    if (nugget) g[d] = log ? -std::log(1.0 + NUGGET_ALPHA_LOWER - alpha) : alpha;
+  }
+  void from_gamma(const double* g, double* theta, double* alpha) const {
+    for (int k = 0; k < d; ++k) theta[k] = log ? std::exp(g[k]) : g[k];
+    if (nugget) *alpha = log ? 1.0 + NUGGET_ALPHA_LOWER - std::exp(-g[d]) : g[d];
+  }
+  // gradient in gamma of -LL from the gradient in (theta, alpha) of LL
+  void to_gamma_grad(const double* theta, double alpha, const double* gll, double* out) const {
+    for (int k = 0; k < d; ++k) out[k] = -(log ? gll[k] * theta[k] : gll[k]);
+    if (nugget) out[d] = -(log ? gll[d] * (1.0 + NUGGET_ALPHA_LOWER - alpha) : gll[d]);
+  }
+};
+
+struct StartWorker {
+  int idx = 0;
+  std::vector<double> theta_start;
+  double extra0 = 0.0;
+  std::vector<double> glo, gup;  // local gamma bounds
+  std::vector<double> gamma;     // current start / optimiser point
+  std::unique_ptr<LBFGSB> opt;
+  int retry = 0;
+  double best_f = std::numeric_limits<double>::infinity();
+  std::vector<double> best_gamma;
+  bool done = false, success = true;
+  std::string error;
+  double objective = std::numeric_limits<double>::infinity();
+};
+}  // namespace
+
+void Kriging::fit_bfgs(int multistart, const Parameters& P) {
+  const OptimConfig& oc = optim_config();
+  const int n = n_, d = d_;
+  const bool nug = noise_ == NoiseModel::Nugget;
+  const int gd = d + (nug ? 1 : 0);
+  std::vector<double> th_lo, th_up;
+  theta_bounds(th_lo, th_up);
+
+  // start points (Kriging.cpp:1708-1765): Random::init(), then
+  // theta_lower + randu(multistart, d) % (upper - lower), column-major fill
+  g_engine.seed(g_seed);
+  std::uniform_real_distribution<double> unif{};
+  std::vector<double> rand0((size_t)multistart * d);  // row-major rows x d
+  for (int k = 0; k < d; ++k)
+    for (int r = 0; r < multistart; ++r) rand0[(size_t)r * d + k] = th_lo[k] + unif(g_engine) * (th_up[k] - th_lo[k]);
+  std::vector<double> theta0;
+  if (P.theta.has_value()) {
+    multistart = std::max(multistart, P.theta_rows);
+    theta0 = *P.theta;
+    theta0.insert(theta0.end(), rand0.begin(), rand0.end());
+    theta0.resize((size_t)multistart * d);
+  } else {
+    theta0 = rand0;
+  }
+  // extra0 (alpha) for the nugget model (Kriging.cpp:1774-1783)
+  std::vector<double> extra0;
+  double ext_lo = 0.0, ext_up = 1.0;
+  if (nug) {
+    ext_lo = NUGGET_ALPHA_LOWER;
+    ext_up = 1.0;
+    if (P.sigma2.has_value() && P.nugget.has_value()) {
+      const double s = *P.sigma2, nu = *P.nugget;
+      extra0.push_back((s > 0 && (s + nu) > 0) ? s / (s + nu) : ext_lo + (ext_up - ext_lo) * 0.5);
+    } else {
+      const int rows = multistart;
+      for (int r = 0; r < rows; ++r) {
+        const double u = unif(g_engine);
+        extra0.push_back(ext_lo + (ext_up - ext_lo) * (1.0 - u * u * u));
+      }
+    }
+  }
+  Reparam rp{d, nug, oc.reparametrize};
+  std::vector<double> glo(gd), gup(gd);
+  {
+    rp.to_gamma(th_lo.data(), ext_lo, glo.data());
+    rp.to_gamma(th_up.data(), ext_up, gup.data());
+  }
+  // estimation flags (Kriging.cpp:1832-1849)
+  est_sigma2_ = P.is_sigma2_estim;
+  if (!est_sigma2_ && P.sigma2.has_value()) {
+    sigma2_ = *P.sigma2;
+    if (normalize_) sigma2_ /= scaleY_ * scaleY_;
+  } else {
+    est_sigma2_ = true;
+  }
+  if (nug) {
+    est_nugget_ = P.is_nugget_estim;
+    if (!est_nugget_ && P.nugget.has_value()) {
+      nugget_ = *P.nugget;
+      if (normalize_) nugget_ /= scaleY_ * scaleY_;
+    } else {
+      est_nugget_ = true;
+    }
+  }
+  const double pgtol = oc.gradient_tolerance;
+  const double factr = oc.objective_rel_tolerance / 1e-13;
+
+  std::vector<StartWorker> W(multistart);
+  auto restart_point = [&](StartWorker& w, const std::vector<double>& th) {
+    w.gamma.assign(gd, 0.0);
+    rp.to_gamma(th.data(), nug ? w.extra0 : 1.0, w.gamma.data());
+    for (int k = 0; k < gd; ++k) {
+      w.glo[k] = std::min(w.gamma[k], w.glo[k]);
+      w.gup[k] = std::max(w.gamma[k], w.gup[k]);
+    }
+    w.opt = std::make_unique<LBFGSB>(gd, oc.lbfgs_memory);
+    w.opt->max_iter = oc.max_iteration;
+    w.opt->pgtol = pgtol;
+    w.opt->factr = factr;
+    w.opt->start(w.gamma, w.glo, w.gup);
+  };
+  for (int s = 0; s < multistart; ++s) {
+    StartWorker& w = W[s];
+    w.idx = s;
+    w.theta_start.assign(theta0.begin() + (size_t)s * d, theta0.begin() + (size_t)(s + 1) * d);
+    w.extra0 = nug ? extra0[s % extra0.size()] : 1.0;
+    w.glo = glo;
+    w.gup = gup;
+    restart_point(w, w.theta_start);
+    w.best_gamma = w.gamma;
+  }
+
+  // objective of one start: -LL in gamma, gradient -dLL/dgamma
+  const ModelFlags flags = flags_for_eval();
+  fit_evals_ = 0;
+  fit_batches_ = 0;
+  std::vector<EvalRequest> reqs(multistart);
+  std::vector<std::vector<double>> th_alpha(multistart, std::vector<double>(gd));
+  std::vector<EvalResult> res;
+  auto eval_pending = [&](bool want_grad, auto&& point_of) {
+    // one batched pipeline over every start that waits for an evaluation;
+    // the others stay in their slots, inactive, so the layout (and the
+    // captured graph) does not change between iterations
+    for (int s = 0; s < multistart; ++s) {
+      const double* g = point_of(W[s]);
+      reqs[s].problem = problem_;
+      reqs[s].gamma_n = gd;
+      reqs[s].flags = flags;
+      reqs[s].active = g != nullptr;
+      if (g != nullptr) {
+        double alpha = 1.0;
+        rp.from_gamma(g, th_alpha[s].data(), &alpha);
+        if (nug) th_alpha[s][d] = alpha;
+      }
+      reqs[s].gamma = th_alpha[s].data();
+    }
+    if (batched_fit_) {
+      rt_->evaluate(reqs, want_grad, res);
+      ++fit_batches_;
+    } else {
+      res.assign(multistart, EvalResult{});
+      std::vector<EvalRequest> one(1);
+      std::vector<EvalResult> r1;
+      for (int s = 0; s < multistart; ++s) {
+        if (!reqs[s].active) continue;
+        one[0] = reqs[s];
+        rt_->evaluate(one, want_grad, r1);
+        res[s] = r1[0];
+        ++fit_batches_;
+      }
+    }
+    for (int s = 0; s < multistart; ++s) fit_evals_ += reqs[s].active ? 1 : 0;
+  };
+
+  std::vector<double> gbuf(gd);
+  for (;;) {
+    bool any = false;
+    for (auto& w : W) any = any || (!w.done && w.opt->need_eval());
+    if (!any) break;
+    eval_pending(true, [](StartWorker& w) -> const double* {
+      return (!w.done && w.opt->need_eval()) ? w.opt->x().data() : nullptr;
+    });
+    for (int s = 0; s < multistart; ++s) {
+      StartWorker& w = W[s];
+      if (!reqs[s].active) continue;
+      if (res[s].status != EVAL_OK) {
+        // the reference's objective throws here and the start is dropped
+        // (Kriging.cpp:2075-2081)
+        w.done = true;
+        w.success = false;
+        w.error = "Exceed max numerical nugget added to force chol matrix";
+        continue;
+      }
+      const double f = -res[s].ll;
+      double alpha = 1.0;
+      std::vector<double> th(d);
+      rp.from_gamma(w.opt->x().data(), th.data(), &alpha);
+      rp.to_gamma_grad(th.data(), alpha, res[s].grad, gbuf.data());
+      w.opt->tell(f, gbuf);
+      if (!w.opt->done()) continue;
+      // one optimiser run finished: restart rules (Kriging.cpp:1994-2041)
+      w.gamma = w.opt->x_opt();
+      const double fopt = w.opt->f_opt();
+      if (fopt < w.best_f) {
+        w.best_f = fopt;
+        w.best_gamma = w.gamma;
+      }
+      std::vector<double> theta_part(d);
+      double a_tmp;
+      rp.from_gamma(w.gamma.data(), theta_part.data(), &a_tmp);
+      double sol_to_lb = std::numeric_limits<double>::infinity();
+      for (int k = 0; k < d; ++k) sol_to_lb = std::min(sol_to_lb, std::fabs(theta_part[k] - th_lo[k]));
+      const bool restart = (w.retry < oc.max_restart) &&
+                           (w.opt->task().rfind("ABNORMAL_TERMINATION_IN_LNSRCH", 0) == 0 ||
+                            w.opt->num_iters() <= 2 || sol_to_lb < std::numeric_limits<double>::epsilon() ||
+                            fopt > w.best_f);
+      if (restart) {
+        std::vector<double> rth(d);
+        const double div = std::pow(2.0, w.retry + 1);
+        for (int k = 0; k < d; ++k) rth[k] = (w.theta_start[k] + th_lo[k]) / div;
+        w.retry++;
+        restart_point(w, rth);
+      } else {
+        w.done = true;
+      }
+    }
+  }
+  // final objective at each start's best point (fit_ofn(best_gamma, nullptr))
+  eval_pending(false, [](StartWorker& w) -> const double* { return w.success ? w.best_gamma.data() : nullptr; });
+  int best = -1, nsucc = 0;
+  double min_ofn = std::numeric_limits<double>::infinity();
+  for (int s = 0; s < multistart; ++s) {
+    StartWorker& w = W[s];
+    if (!w.success) continue;
+    if (res[s].status != EVAL_OK) {
+      w.success = false;
+      continue;
+    }
+    w.objective = -res[s].ll;
+    ++nsucc;
+    if (w.objective < min_ofn) {
+      min_ofn = w.objective;
+      best = s;
+    }
+  }
+  if (nsucc == 0) throw std::runtime_error("All " + std::to_string(multistart) + " optimization attempts failed");
+  std::vector<double> th(d);
+  double alpha = 1.0;
+  rp.from_gamma(W[best].best_gamma.data(), th.data(), &alpha);
+  est_theta_ = true;
+  alpha_ = alpha;
+  commit(th, alpha);
+}
+
+// ---------------------------------------------------------------------------
+// predict (Kriging::predict, Kriging.cpp:2241-2285 -> predict_impl,
+// KrigingImpl.cpp:145-243): mean and standard deviation at m points
+// ---------------------------------------------------------------------------
+void Kriging::predict(const double* Xn, int m, bool want_stdev, double* mean, double* stdev) {
+  if (is_empty_) throw std::runtime_error("model is not fitted");
+  const int d = d_, p = p_;
+  if (m <= 0) return;
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  const int chunk = std::min(m, 8192);
+  const int mpad_max = (chunk + NB - 1) / NB * NB;
+  double *dXn = nullptr, *dRon = nullptr, *dC = nullptr, *dout = nullptr, *dcoef = nullptr;
+  auto cleanup = [&] {
+    if (dXn) hipFree(dXn);
+    if (dRon) hipFree(dRon);
+    if (dC) hipFree(dC);
+    if (dout) hipFree(dout);
+    if (dcoef) hipFree(dcoef);
+  };
+  try {
+    hip_check(hipMalloc(&dXn, sizeof(double) * (size_t)chunk * d), "hipMalloc");
+    hip_check(hipMalloc(&dRon, sizeof(double) * (size_t)npad_ * mpad_max), "hipMalloc");
+    hip_check(hipMalloc(&dC, sizeof(double) * (size_t)npad_ * mpad_max), "hipMalloc");
+    hip_check(hipMalloc(&dout, sizeof(double) * 2 * (size_t)chunk), "hipMalloc");
+    hip_check(hipMalloc(&dcoef, sizeof(double) * (size_t)(p + p * p + 1)), "hipMalloc");
+    std::vector<double> coef(p + p * p);
+    for (int a = 0; a < p; ++a) coef[a] = beta_[a];
+    for (int q = 0; q < p * p; ++q) coef[p + q] = Rstar_[q];
+    if (p > 0) hip_check(hipMemcpy(dcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice), "upload");
+    const bool nug = noise_ == NoiseModel::Nugget;
+    const double var_scale = nug ? sigma2_ / alpha_ : sigma2_;
+    std::vector<double> xn((size_t)chunk * d), host_out(2 * (size_t)chunk);
+    for (int s0 = 0; s0 < m; s0 += chunk) {
+      const int mc = std::min(chunk, m - s0);
+      for (int k = 0; k < d; ++k)
+        for (int j = 0; j < mc; ++j) xn[j + (size_t)k * mc] = (Xn[s0 + j + (size_t)k * m] - centerX_[k]) / scaleX_[k];
+      hip_check(hipMemcpy(dXn, xn.data(), sizeof(double) * (size_t)mc * d, hipMemcpyHostToDevice), "upload Xn");
+      PredictArgs a{};
+      a.X = rt_->problem(problem_).X;
+      a.Linv = d_Linv_;
+      a.MZ = d_vec_;
+      a.coef = dcoef;
+      a.Xn = dXn;
+      a.Ron = dRon;
+      a.C = dC;
+      a.out = dout;
+      for (int k = 0; k < d; ++k) a.theta[k] = theta_[k];
+      a.factor = nug ? alpha_ : 1.0;
+      a.n = n_;
+      a.d = d;
+      a.npad = npad_;
+      a.p = p;
+      a.m = mc;
+      a.mpad = (mc + NB - 1) / NB * NB;
+      a.kernel = kernel_id(kernel_);
+      a.linear_trend = regmodel_ == RegModel::Linear;
+      hip_check(launch_predict(a, rt_->stream()), "predict");
+      hip_check(hipMemcpyAsync(host_out.data(), dout, sizeof(double) * 2 * mc, hipMemcpyDeviceToHost, rt_->stream()),
+                "download");
+      hip_check(hipStreamSynchronize(rt_->stream()), "sync");
+      for (int j = 0; j < mc; ++j) {
+        mean[s0 + j] = centerY_ + scaleY_ * host_out[j];
+        if (want_stdev && stdev != nullptr)
+          stdev[s0 + j] = std::sqrt(host_out[mc + j] * var_scale * scaleY_ * scaleY_);
+      }
+    }
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
+}
+
+// Evaluate the model at (theta, alpha), keep L^-1, z, M, R* on the device
+// for predict, and set sigma2 / nugget / beta as the reference's commit
+// (Kriging.cpp:2156-2202 and the optim="none" branch 1681-1700).
+void Kriging::commit(const std::vector<double>& theta, double alpha) {
+  const int d = d_, p = p_;
+  std::vector<double> g(theta);
+  if (noise_ == NoiseModel::Nugget) g.push_back(alpha);
+  EvalRequest R;
+  R.problem = problem_;
+  R.gamma = g.data();
+  R.gamma_n = (int)g.size();
+  R.flags = flags_for_eval();
+  R.flags.alpha = alpha;
+  std::vector<EvalRequest> reqs{R};
+  std::vector<EvalResult> res;
+  rt_->evaluate(reqs, false, res);
+  if (res[0].status != EVAL_OK)
+    throw std::runtime_error("[ERROR] Exceed max numerical nugget added to force chol matrix");
+  theta_ = theta;
+  const Runtime::SlotView v = rt_->slot_view(0);
+  npad_ = v.npad;
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  if (d_Linv_) hipFree(d_Linv_);
+  if (d_vec_) hipFree(d_vec_);
+  d_Linv_ = nullptr;
+  d_vec_ = nullptr;
+  hip_check(hipMalloc(&d_Linv_, sizeof(double) * (size_t)npad_ * npad_), "hipMalloc model");
+  hip_check(hipMalloc(&d_vec_, sizeof(double) * (size_t)npad_ * (p + 1)), "hipMalloc model");
+  hip_check(hipMemcpyAsync(d_Linv_, v.W, sizeof(double) * (size_t)npad_ * npad_, hipMemcpyDeviceToDevice,
+                           rt_->stream()),
+            "copy model");
+  if (p > 0)
+    hip_check(hipMemcpyAsync(d_vec_, v.V, sizeof(double) * (size_t)npad_ * p, hipMemcpyDeviceToDevice, rt_->stream()),
+              "copy model");
+  const double sse = res[0].sse;
+  std::vector<double> bhat(res[0].beta, res[0].beta + p);
+  if (est_beta_) {
+    beta_ = bhat;
+    hip_check(hipMemcpyAsync(d_vec_ + (size_t)npad_ * p, v.V + (size_t)(p + 2) * npad_, sizeof(double) * npad_,
+                             hipMemcpyDeviceToDevice, rt_->stream()),
+              "copy model");
+  } else {
+    // z = y* - M beta (Kriging.cpp:2170-2172)
+    std::vector<double> ystar(n_), M((size_t)n_ * std::max(p, 1)), z(npad_, 0.0);
+    rt_->read_slot(0, 3, ystar.data());
+    if (p > 0) rt_->read_slot(0, 2, M.data());
+    for (int i = 0; i < n_; ++i) {
+      double s = ystar[i];
+      for (int a = 0; a < p; ++a) s -= M[i + (size_t)a * n_] * beta_[a];
+      z[i] = s;
+    }
+    hip_check(hipMemcpyAsync(d_vec_ + (size_t)npad_ * p, z.data(), sizeof(double) * npad_, hipMemcpyHostToDevice,
+                             rt_->stream()),
+              "upload z");
+  }
+  Rstar_.assign((size_t)p * p, 0.0);
+  if (p > 0) rt_->read_slot(0, 6, Rstar_.data());
+  hip_check(hipStreamSynchronize(rt_->stream()), "sync");
+  const int n = n_;
+  const double total_var = sse / n;
+  if (noise_ == NoiseModel::Nugget) {
+    if (!est_theta_) {
+      // optim = "none" (Kriging.cpp:1688-1696); nugget_ holds the given value
+      if (est_sigma2_) {
+        sigma2_ = alpha * total_var;
+        nugget_ = est_nugget_ ? (1.0 - alpha) * total_var : nugget_;
+      } else {
+        nugget_ = est_nugget_ ? 0.0 : nugget_;
+      }
+    } else if (est_sigma2_) {
+      // BFGS commit (Kriging.cpp:2174-2189)
+      if (est_nugget_) {
+        sigma2_ = alpha * total_var;
+        nugget_ = sigma2_ / alpha - sigma2_;
+      } else {
+        sigma2_ = nugget_ * alpha / (1.0 - alpha);
+      }
+    } else if (est_nugget_) {
+      nugget_ = sigma2_ * (1.0 - alpha) / alpha;
+    }
+  } else if (est_sigma2_) {
+    sigma2_ = total_var;
+  }
+  is_empty_ = false;
+  (void)d;
+}
+
+}  // namespace lk

@@ -1,0 +1,149 @@
+// Kriging model (ordinary / universal kriging, optional nugget) on the GPU
+// likelihood pipeline.  Mirrors the reference class Kriging
+// (src/lib/include/libKriging/Kriging.hpp:18-366): constructor, fit (BFGS
+// multistart with restarts, or "none"), logLikelihoodFun, logLikelihood,
+// predict and the getters, with the same argument meaning.  All the
+// O(n^2)..O(n^3) work runs through lk::Runtime; the host keeps the
+// optimiser state machines and the O(d) parameter transforms.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <optional>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "lk_runtime.hpp"
+
+namespace lk {
+
+enum class NoiseModel { None = 0, Nugget = 1 };
+enum class RegModel { None = 0, Constant = 1, Linear = 2 };
+
+RegModel regmodel_from_string(const std::string& s);
+int kernel_id(const std::string& name);
+
+// KrigingParameters (Kriging.hpp:18-28); theta is rows x d, row-major
+struct Parameters {
+  std::optional<double> sigma2;
+  bool is_sigma2_estim = true;
+  std::optional<std::vector<double>> theta;
+  int theta_rows = 0;
+  bool is_theta_estim = true;
+  std::optional<std::vector<double>> beta;
+  bool is_beta_estim = true;
+  std::optional<double> nugget;
+  bool is_nugget_estim = true;
+};
+
+// Process-global optimisation knobs (Optim statics, src/lib/Optim.cpp:16-141)
+struct OptimConfig {
+  bool reparametrize = true;
+  double theta_lower_factor = 0.02;
+  double theta_upper_factor = 10.0;
+  bool variogram_bounds_heuristic = true;
+  int max_restart = 10;
+  int max_iteration = 20;
+  double gradient_tolerance = 1e-3;
+  double objective_rel_tolerance = 1e-3;
+  int lbfgs_memory = 5;
+  int log_level = 0;
+};
+OptimConfig& optim_config();
+
+// Random (src/lib/Random.cpp): global mt19937 reseeded by every BFGS fit
+void random_reset_seed(unsigned seed);
+
+// Per-process runtime of one device (created on first use)
+Runtime& runtime_for(int device);
+int default_device();
+
+class Kriging {
+ public:
+  Kriging(const std::string& kernel, NoiseModel noise = NoiseModel::None);
+  ~Kriging();
+  Kriging(const Kriging&) = delete;
+  Kriging& operator=(const Kriging&) = delete;
+
+  // X: n x d column-major
+  void fit(const double* y, const double* X, int n, int d, RegModel regmodel = RegModel::Constant,
+           bool normalize = false, const std::string& optim = "BFGS", const std::string& objective = "LL",
+           const Parameters& parameters = Parameters{});
+
+  // gamma = theta (d) or [theta, alpha] (d + 1, nugget model); returns LL
+  double logLikelihoodFun(const double* gamma, int gamma_n, bool want_grad, double* grad);
+  // nb evaluations at once (rows of gamma, each gamma_n long), one batched pipeline
+  void logLikelihoodFunBatch(int nb, const double* gamma, int gamma_n, bool want_grad, double* ll, double* grad);
+  double logLikelihood();
+
+  // Xn: m x d column-major; mean[m], stdev[m] (stdev may be null)
+  void predict(const double* Xn, int m, bool want_stdev, double* mean, double* stdev);
+
+  // getters (normalised space, as the reference's members)
+  const std::string& kernel() const { return kernel_; }
+  NoiseModel noise() const { return noise_; }
+  const std::vector<double>& theta() const { return theta_; }
+  double sigma2() const { return sigma2_; }
+  double nugget() const { return nugget_; }
+  double alpha() const { return alpha_; }
+  const std::vector<double>& beta() const { return beta_; }
+  int n() const { return n_; }
+  int d() const { return d_; }
+  int p() const { return p_; }
+  bool is_fitted() const { return !is_empty_; }
+  const std::string& optim() const { return optim_; }
+  const std::string& objective() const { return objective_; }
+  double centerY() const { return centerY_; }
+  double scaleY() const { return scaleY_; }
+  const std::vector<double>& centerX() const { return centerX_; }
+  const std::vector<double>& scaleX() const { return scaleX_; }
+  bool normalize() const { return normalize_; }
+  RegModel regmodel() const { return regmodel_; }
+  const std::vector<double>& X() const { return X_; }
+  const std::vector<double>& y() const { return y_; }
+  // statistics of the last fit
+  int fit_evaluations() const { return fit_evals_; }
+  int fit_batches() const { return fit_batches_; }
+  // force the plain one-evaluation-per-launch path in fit (comparison only)
+  void set_batched_fit(bool b) { batched_fit_ = b; }
+
+ private:
+  void setup(const double* y, const double* X, int n, int d, RegModel regmodel, bool normalize);
+  ModelFlags flags_for_eval() const;
+  void commit(const std::vector<double>& theta, double alpha);
+  void theta_bounds(std::vector<double>& lo, std::vector<double>& up) const;
+  void fit_bfgs(int multistart, const Parameters& parameters);
+
+  std::string kernel_;
+  NoiseModel noise_;
+  int device_;
+  Runtime* rt_ = nullptr;
+  int problem_ = -1;
+
+  // data (normalised), column-major
+  int n_ = 0, d_ = 0, p_ = 0;
+  std::vector<double> X_, y_, F_;
+  RegModel regmodel_ = RegModel::Constant;
+  bool normalize_ = false;
+  std::vector<double> centerX_, scaleX_;
+  double centerY_ = 0.0, scaleY_ = 1.0;
+  std::string optim_ = "BFGS", objective_ = "LL";
+
+  // fitted state
+  bool is_empty_ = true;
+  std::vector<double> theta_;
+  double sigma2_ = 1.0, nugget_ = 0.0, alpha_ = 1.0;
+  bool est_sigma2_ = true, est_nugget_ = true, est_beta_ = true, est_theta_ = true;
+  std::vector<double> beta_;
+
+  // committed model on the device: L^-1 (npad x npad), z = E*, M = F*, R*
+  double* d_Linv_ = nullptr;
+  double* d_vec_ = nullptr;  // npad x (p + 1): [M | z]
+  int npad_ = 0;
+  std::vector<double> Rstar_;  // p x p upper
+
+  int fit_evals_ = 0, fit_batches_ = 0;
+  bool batched_fit_ = true;
+};
+
+}  // namespace lk

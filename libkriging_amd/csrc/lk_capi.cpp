@@ -115,6 +115,28 @@ int lkgpu_ll_grad_batch(lkgpu_context* ctx, int nb, const int* problem_ids, cons
   });
 }
 
+int lkgpu_synchronize(lkgpu_context* ctx) {
+  return guarded([&] {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    lk::hip_check(hipSetDevice(ctx->rt.device()), "hipSetDevice");
+    lk::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  });
+}
+
+int lkgpu_set_timed_stage(lkgpu_context* ctx, int stage) {
+  return guarded([&] {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    ctx->rt.set_timed_stage(stage);
+  });
+}
+
+int lkgpu_take_stage_time(lkgpu_context* ctx, double* avg_ms, int* count) {
+  return guarded([&] {
+    if (ctx == nullptr || avg_ms == nullptr || count == nullptr) throw std::invalid_argument("NULL argument");
+    ctx->rt.take_stage_time(avg_ms, count);
+  });
+}
+
 int lkgpu_read_item(lkgpu_context* ctx, int item, int what, double* out) {
   return guarded([&] {
     if (ctx == nullptr || out == nullptr) throw std::invalid_argument("NULL argument");

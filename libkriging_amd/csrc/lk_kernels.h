@@ -78,9 +78,9 @@ __host__ __device__ inline int red_colL_off(int nblk) { return 2 * nblk; }     /
 __host__ __device__ inline int red_colW_off(int nblk) { return 3 * nblk; }     // nblk partial |L^-1|_1 column maxima
 __host__ __device__ inline int red_grad_off(int nblk) { return 4 * nblk; }     // tiles x GRAD_STRIDE adjoint partials
 constexpr int GRAD_STRIDE = 2 * DMAX + 4;
-__host__ __device__ inline size_t red_size(int nblk) {
-  return (size_t)4 * nblk + (size_t)nblk * (nblk + 1) / 2 * GRAD_STRIDE + 64;
-}
+// R* = chol_upper(F*'F*), p x p column-major, after the adjoint partials
+__host__ __device__ inline int red_rstar_off(int nblk) { return 4 * nblk + nblk * (nblk + 1) / 2 * GRAD_STRIDE; }
+__host__ __device__ inline size_t red_size(int nblk) { return (size_t)red_rstar_off(nblk) + PMAX * PMAX + 64; }
 
 struct Batch {
   const Slot* slots;        // device
@@ -108,5 +108,23 @@ hipError_t launch_finalize_grad(const Batch& b);
 hipError_t launch_rcond_estimate(const Batch& b);
 
 int trtri_levels(int nblk);
+
+// Prediction at new points from a committed model (KrigingImpl::predict_impl,
+// src/lib/KrigingImpl.cpp:145-243): mean and variance in normalised units.
+struct PredictArgs {
+  const double* X;     // n x d design (normalised), ld = n
+  const double* Linv;  // npad x npad, lower triangle = L^-1
+  const double* MZ;    // npad x (p + 1): [M = F* | z]
+  const double* coef;  // beta (p) then R* (p x p, column-major upper)
+  const double* Xn;    // m x d new points (normalised), ld = m
+  double* Ron;         // npad x mpad workspace: R(x_obs, x_new)
+  double* C;           // npad x mpad workspace: L^-1 R_on
+  double* out;         // 2 m: mean (normalised), variance factor
+  double theta[DMAX];
+  double factor;       // R_on_factor (alpha for the nugget model)
+  int n, d, npad, p, m, mpad;
+  int kernel, linear_trend;
+};
+hipError_t launch_predict(const PredictArgs& a, hipStream_t stream);
 
 }  // namespace lk

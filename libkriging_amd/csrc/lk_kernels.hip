@@ -284,73 +284,224 @@ __global__ __launch_bounds__(TPB) void k_build(const Slot* __restrict__ slots, c
 }
 
 // ---------------------------------------------------------------------------
-// (2a) Cholesky of the diagonal block k and its inverse (one wave per slot).
-// Lane l holds row l of the block in registers; pivots and columns are
-// broadcast with v_readlane.  Writes L_kk to A (zero strict upper part) and
-// L_kk^-1 to the same block of W, the per-block partial log-det, and flags a
-// non-positive / NaN pivot (dpotrf failure, LinearAlgebra.cpp:71).
+// (2a) Cholesky of the diagonal block k and its inverse, one 256-thread
+// workgroup per slot.  Lane l is row l (factor) or column l (inverse); wave w
+// owns columns (factor) or rows (inverse) 16w .. 16w+15 in registers.  Both
+// sweeps go 4 columns / rows at a time: a 4 x 4 diagonal sub-block is
+// factored redundantly by every lane from LDS broadcasts, then a rank-4
+// update; 16 sequential steps per sweep instead of 64.  Pivot square roots
+// come from v_rsq_f64 refined by two Newton steps.  Writes L_kk to A and
+// L_kk^-1 to the same block of W (both with zero strict upper part), the
+// partial log-det, and flags a non-positive / NaN pivot (dpotrf failure,
+// LinearAlgebra.cpp:71).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_potrf_diag(const Slot* __restrict__ slots,
-                                                   const SlotParams* __restrict__ params,
-                                                   SlotOut* __restrict__ out, int k) {
+__device__ inline void pivot_sqrt(double a, double& root, double& rroot) {
+  double y = __builtin_amdgcn_rsq(a);
+  const double h = 0.5 * a;
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double r = fma(-h * y, y, 0.5);
+    y = fma(y, r, y);
+  }
+  double s = a * y;
+  const double e = fma(-s, s, a);
+  s = fma(e, 0.5 * y, s);
+  root = s;
+  rroot = y;
+}
+
+struct DiagLds {
+  double P[2][NB * 4];  // current values of 4 columns, row-major [row][4]
+  double Q[2][NB * 4];  // their L values
+  double L[NB * (NB + 1)];  // L_kk column-major, stride NB + 1
+  double X[NB * NB];        // L_kk^-1 rows staged for coalesced stores
+  double R[2][4 * NB];      // 4 rows of L_kk^-1 of the current inverse step
+  double diag[NB], rinv[NB];
+};
+
+// factor sweep for wave W (owns columns 16W .. 16W+15)
+template <int W>
+__device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
+  bool ok = true;
+#pragma unroll
+  for (int p = 0; p < 16; ++p) {
+    const int j0 = 4 * p;
+    double* P = sh.P[p & 1];
+    double* Q = sh.Q[p & 1];
+    if (W == (p >> 2)) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) P[l * 4 + s] = a[(j0 & 15) + s];
+    }
+    __syncthreads();
+    // 4 x 4 diagonal sub-block, factored by every lane
+    double b[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c <= r; ++c) b[r][c] = P[(j0 + r) * 4 + c];
+    double L4[4][4], iv[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double d = b[c][c];
+#pragma unroll
+      for (int m = 0; m < c; ++m) d = fma(-L4[c][m], L4[c][m], d);
+      ok = ok && (d > 0.0);
+      pivot_sqrt(d, L4[c][c], iv[c]);
+#pragma unroll
+      for (int r = c + 1; r < 4; ++r) {
+        double t = b[r][c];
+#pragma unroll
+        for (int m = 0; m < c; ++m) t = fma(-L4[r][m], L4[c][m], t);
+        L4[r][c] = t * iv[c];
+      }
+    }
+    // this row's L values in columns j0 .. j0+3
+    double x[4];
+    {
+      double v[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) v[s] = P[l * 4 + s];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        double t = v[s];
+#pragma unroll
+        for (int m = 0; m < s; ++m) t = fma(-x[m], L4[s][m], t);
+        x[s] = t * iv[s];
+      }
+      if (l < j0 + 4) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          double vv = 0.0;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (l == j0 + r && s <= r) vv = L4[r][s];
+          x[s] = (l < j0) ? 0.0 : vv;
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) Q[l * 4 + s] = x[s];
+    if (W == (p >> 2)) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) a[(j0 & 15) + s] = x[s];
+    }
+    if (W == 0 && l == 0) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        sh.diag[j0 + s] = L4[s][s];
+        sh.rinv[j0 + s] = iv[s];
+      }
+    }
+    __syncthreads();
+    // rank-4 update of this wave's columns right of the block
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int c = 16 * W + q;
+      if (c >= j0 + 4) {
+        double t = a[q];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) t = fma(-x[s], Q[c * 4 + s], t);
+        a[q] = t;
+      }
+    }
+  }
+  return ok;
+}
+
+// inverse sweep for wave W (owns rows 16W .. 16W+15 of X = L_kk^-1; lane = column)
+template <int W>
+__device__ inline void diag_inverse(DiagLds& sh, int l) {
+  double acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+#pragma unroll
+  for (int p = 0; p < 16; ++p) {
+    const int j0 = 4 * p;
+    double* Rr = sh.R[p & 1];
+    if (W == (p >> 2)) {
+      double x[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        double t = ((l == j0 + s) ? 1.0 : 0.0) - acc[(j0 & 15) + s];
+#pragma unroll
+        for (int m = 0; m < s; ++m) t = fma(-sh.L[(j0 + m) * (NB + 1) + j0 + s], x[m], t);
+        x[s] = t * sh.rinv[j0 + s];
+        Rr[s * NB + l] = x[s];
+        sh.X[(j0 + s) * NB + l] = x[s];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int i = 16 * W + q;
+      if (i >= j0 + 4) {
+        double t = acc[q];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) t = fma(sh.L[(j0 + s) * (NB + 1) + i], Rr[s * NB + l], t);
+        acc[q] = t;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_potrf_diag(const Slot* __restrict__ slots,
+                                                    const SlotParams* __restrict__ params,
+                                                    SlotOut* __restrict__ out, int k) {
   const int b = blockIdx.x;
   const Slot S = slots[b];
   if (!slot_live(params[b], out[b]) || k >= S.nblk) return;
-  const int l = threadIdx.x;
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
   const int k0 = k * NB;
   const int ld = S.npad;
   double* Ab = S.A + (size_t)k0 * ld + k0;
   double* Wb = S.W + (size_t)k0 * ld + k0;
+  __shared__ DiagLds sh;
+  __shared__ int sbad;
+  if (tid == 0) sbad = 0;
 
-  double a[NB];
+  double a[16];
 #pragma unroll
-  for (int c = 0; c < NB; ++c) a[c] = Ab[(size_t)c * ld + l];
-
-  __shared__ double sL[NB * (NB + 1)];
-  __shared__ double sDiag[NB];
-  bool ok = true;
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    const double ajj = readlane_d(a[j], j);
-    ok = ok && (ajj > 0.0);
-    const double ljj = sqrt(ajj);
-    const double rinv = 1.0 / ljj;
-    a[j] = (l == j) ? ljj : ((l > j) ? a[j] * rinv : 0.0);
-#pragma unroll
-    for (int c = j + 1; c < NB; ++c) {
-      const double lcj = readlane_d(a[j], c);
-      a[c] = fma(-a[j], lcj, a[c]);
-    }
-    if (l == 0) sDiag[j] = ljj;
+  for (int q = 0; q < 16; ++q) a[q] = Ab[(size_t)(16 * w + q) * ld + l];
+  bool ok;
+  switch (w) {
+    case 0: ok = diag_factor<0>(a, sh, l); break;
+    case 1: ok = diag_factor<1>(a, sh, l); break;
+    case 2: ok = diag_factor<2>(a, sh, l); break;
+    default: ok = diag_factor<3>(a, sh, l); break;
   }
+  // every lane saw every pivot, so ok is uniform; no early exit before the
+  // sweeps' barriers
   if (!ok) {
-    if (l == 0) out[b].status = ST_CHOL_FAILED;
+    if (tid == 0) out[b].status = ST_CHOL_FAILED;
     return;
   }
-  // L_kk -> A (lower, zero above) and LDS (column-major, stride NB + 1)
+  // L_kk -> A (lower, zero above) and LDS
 #pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    const double v = (l >= c) ? a[c] : 0.0;
+  for (int q = 0; q < 16; ++q) {
+    const int c = 16 * w + q;
+    const double v = (l >= c) ? a[q] : 0.0;
     Ab[(size_t)c * ld + l] = v;
-    sL[c * (NB + 1) + l] = v;
+    sh.L[c * (NB + 1) + l] = v;
   }
   __syncthreads();
-  // log-det partial
-  {
-    const double ld_part = wave_sum(log(sDiag[l]));
+  if (w == 0) {
+    const double ld_part = wave_sum(log(sh.diag[l]));
     if (l == 0) S.red[RED_LOGDET + k] = ld_part;
   }
-  // row l of L_kk^-1: X(l, c) = (delta_lc - sum_{m > c} X(l, m) L(m, c)) / L(c, c)
-  double x[NB];
-#pragma unroll
-  for (int c = NB - 1; c >= 0; --c) {
-    double s = (l == c) ? 1.0 : 0.0;
-#pragma unroll
-    for (int m = c + 1; m < NB; ++m) s = fma(-x[m], sL[c * (NB + 1) + m], s);
-    x[c] = s / sL[c * (NB + 1) + c];
+  switch (w) {
+    case 0: diag_inverse<0>(sh, l); break;
+    case 1: diag_inverse<1>(sh, l); break;
+    case 2: diag_inverse<2>(sh, l); break;
+    default: diag_inverse<3>(sh, l); break;
   }
+  __syncthreads();
+  // X rows staged as sh.X[row][col]; write columns coalesced (lane = row)
 #pragma unroll
-  for (int c = 0; c < NB; ++c) Wb[(size_t)c * ld + l] = x[c];
+  for (int q = 0; q < 16; ++q) {
+    const int c = 16 * w + q;
+    Wb[(size_t)c * ld + l] = sh.X[l * NB + c];
+  }
+  (void)sbad;
 }
 
 // ---------------------------------------------------------------------------
@@ -575,6 +726,9 @@ __global__ __launch_bounds__(TPB) void k_gls(const Slot* __restrict__ slots, con
       beta[a] = z[a];
       out[b].beta[a] = z[a];
     }
+    double* rs = S.red + red_rstar_off(S.nblk);
+    for (int c = 0; c < p; ++c)
+      for (int a = 0; a < p; ++a) rs[a + c * p] = (a <= c) ? G[a + c * PMAX] : 0.0;
     if (!ok) out[b].status = ST_CHOL_FAILED;
   }
   __syncthreads();
@@ -1073,6 +1227,131 @@ __global__ __launch_bounds__(TPB) void k_rcond_estimate(const Slot* __restrict__
 // ---------------------------------------------------------------------------
 static inline int tri(int t) { return t * (t + 1) / 2; }
 
+// ---------------------------------------------------------------------------
+// (9) prediction: R_on, C = L^-1 R_on, per-point reductions
+// ---------------------------------------------------------------------------
+// R_on(i, j) = factor * k(x_i - xn_j), 1 where the points coincide
+// (|dx_k| <= eps for all k, KrigingImpl.cpp:202-205); zero padding.
+// grid (mpad / 64, npad / 64), 256 threads
+template <int DT>
+__global__ __launch_bounds__(TPB) void k_cross_cov(PredictArgs a) {
+  const int j0 = blockIdx.x * NB, i0 = blockIdx.y * NB, tid = threadIdx.x;
+  const int n = a.n, m = a.m, d = a.d;
+  __shared__ double sXo[DMAX * NB];
+  __shared__ double sXn[DMAX * NB];
+  for (int q = tid; q < d * NB; q += TPB) {
+    const int k = q / NB, r = q % NB;
+    sXo[q] = (i0 + r < n) ? a.X[(size_t)k * n + i0 + r] : 0.0;
+    sXn[q] = (j0 + r < m) ? a.Xn[(size_t)k * m + j0 + r] : 0.0;
+  }
+  double it[DT];
+#pragma unroll
+  for (int k = 0; k < DT; ++k) it[k] = (k < d) ? 1.0 / a.theta[k] : 1.0;
+  __syncthreads();
+  const int ii = tid & 63, gi = i0 + ii;
+  double dx[DT];
+  for (int r = 0; r < NB / 4; ++r) {
+    const int jj = (tid >> 6) + 4 * r, gj = j0 + jj;
+    double v = 0.0;
+    if (gi < n && gj < m) {
+      bool same = true;
+#pragma unroll
+      for (int k = 0; k < DT; ++k) {
+        dx[k] = (k < d) ? sXo[k * NB + ii] - sXn[k * NB + jj] : 0.0;
+        same = same && fabs(dx[k]) <= 2.220446049250313e-16;
+      }
+      v = same ? 1.0 : kern_rho<DT>(a.kernel, dx, it) * a.factor;
+    }
+    a.Ron[(size_t)gj * a.npad + gi] = v;
+  }
+}
+
+// C(it, jt) = sum_{kt <= it} Linv(it, kt) R_on(kt, jt); grid (mpad / 64, npad / 64)
+__global__ __launch_bounds__(TPB) void k_trmm_lower(PredictArgs a) {
+  const int jt = blockIdx.x, it = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  const int i0 = it * NB, j0 = jt * NB, ld = a.npad;
+  __shared__ double sA[KC * LDSS];
+  __shared__ double sB[KC * LDSS];
+  d4 acc[2][2];
+  acc_zero(acc);
+  for (int kt = 0; kt <= it; ++kt)
+    for (int kc = 0; kc < NB; kc += KC) {
+      stage<false>(sA, a.Linv, ld, i0, kt * NB + kc, tid);  // Linv(i0 + ii, k)
+      stage<true>(sB, a.Ron, ld, j0, kt * NB + kc, tid);    // R_on(k, j0 + jj)
+      __syncthreads();
+      mma_chunk<false>(sA, sB, acc, lane, wr, wc);
+      __syncthreads();
+    }
+  acc_store(acc, a.C + (size_t)j0 * ld + i0, ld, lane, wr, wc);
+}
+
+// One wave per new point j: mean = f_j beta + C(:, j)' z, and
+// 1 - |C(:, j)|^2 + |E circ_j|^2 with E circ_j = (f_j - C(:, j)' M) R*^-1
+// (KrigingImpl.cpp:227-242), clipped at 0.  grid ceil(m / 4), 256 threads
+__global__ __launch_bounds__(TPB) void k_predict_reduce(PredictArgs a) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 4 + w;
+  __shared__ double sfh[4][PMAX];
+  if (j >= a.m) return;
+  const int n = a.n, ld = a.npad, p = a.p;
+  const double* c = a.C + (size_t)j * ld;
+  const double* z = a.MZ + (size_t)p * ld;
+  double mz = 0.0, ss = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double v = c[i];
+    mz = fma(v, z[i], mz);
+    ss = fma(v, v, ss);
+  }
+  mz = wave_sum(mz);
+  ss = wave_sum(ss);
+  for (int q = 0; q < p; ++q) {
+    double s = 0.0;
+    for (int i = lane; i < n; i += 64) s = fma(c[i], a.MZ[(size_t)q * ld + i], s);
+    s = wave_sum(s);
+    if (lane == 0) sfh[w][q] = s;
+  }
+  if (lane != 0) return;
+  const double* beta = a.coef;
+  const double* Rs = a.coef + p;
+  double mean = mz, ee = 0.0;
+  double e[PMAX];
+  for (int q = 0; q < p; ++q) {
+    const double f = (q == 0) ? 1.0 : a.Xn[(size_t)(q - 1) * a.m + j];
+    mean = fma(f, beta[q], mean);
+    e[q] = f - sfh[w][q];
+  }
+  for (int q = 0; q < p; ++q) {  // R*^T c = e (forward substitution)
+    double t = e[q];
+    for (int r = 0; r < q; ++r) t -= Rs[r + q * p] * e[r];
+    e[q] = t / Rs[q + q * p];
+    ee = fma(e[q], e[q], ee);
+  }
+  double v = 1.0 - ss + ee;
+  if (isnan(v) || v < 0.0) v = 0.0;
+  a.out[j] = mean;
+  a.out[a.m + j] = v;
+}
+
+hipError_t launch_predict(const PredictArgs& a, hipStream_t stream) {
+  dim3 grid(a.mpad / NB, a.npad / NB);
+  if (a.d <= 4)
+    hipLaunchKernelGGL(k_cross_cov<4>, grid, dim3(TPB), 0, stream, a);
+  else if (a.d <= 8)
+    hipLaunchKernelGGL(k_cross_cov<8>, grid, dim3(TPB), 0, stream, a);
+  else if (a.d <= 16)
+    hipLaunchKernelGGL(k_cross_cov<16>, grid, dim3(TPB), 0, stream, a);
+  else
+    hipLaunchKernelGGL(k_cross_cov<32>, grid, dim3(TPB), 0, stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_trmm_lower, grid, dim3(TPB), 0, stream, a);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_predict_reduce, dim3((a.m + 3) / 4), dim3(TPB), 0, stream, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_build(const Batch& B) {
   dim3 grid(tri(B.nblk_max), B.nslots);
   if (B.dmax <= 4)
@@ -1086,7 +1365,7 @@ hipError_t launch_build(const Batch& B) {
   return hipGetLastError();
 }
 hipError_t launch_potrf_diag(const Batch& B, int k) {
-  hipLaunchKernelGGL(k_potrf_diag, dim3(B.nslots), dim3(64), 0, B.stream, B.slots, B.params, B.out, k);
+  hipLaunchKernelGGL(k_potrf_diag, dim3(B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, k);
   return hipGetLastError();
 }
 hipError_t launch_potrf_panel(const Batch& B, int k) {

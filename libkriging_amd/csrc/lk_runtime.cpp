@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <limits>
 
 namespace lk {
 
@@ -22,6 +23,8 @@ Runtime::Runtime(int device) : device_(device) {
 
 Runtime::~Runtime() {
   hipSetDevice(device_);
+  for (auto& e : tev_)
+    if (e) hipEventDestroy(e);
   for (auto& g : graphs_)
     if (g.exec) hipGraphExecDestroy(g.exec);
   for (auto& kv : problems_) {
@@ -158,28 +161,81 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
   ++layout_version_;
 }
 
-void Runtime::launch_all(const Batch& B, bool want_grad) {
-  hip_check(launch_build(B), "build");
+// The pipeline as an ordered list of launches, each tagged with the stages
+// it belongs to (bit s = stage s): 0 build, 1 Cholesky, 2 first trailing
+// update, 3 trtri, 4 triangular solves + LL, 5 fused R^-1 / adjoint.
+std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_grad) const {
+  std::vector<Step> v;
+  v.push_back({1u << 0, "build", [B] { return launch_build(B); }});
   for (int k = 0; k < B.nblk_max; ++k) {
-    hip_check(launch_potrf_diag(B, k), "potrf_diag");
-    hip_check(launch_potrf_panel(B, k), "potrf_panel");
-    hip_check(launch_potrf_update(B, k), "potrf_update");
+    v.push_back({1u << 1, "potrf_diag", [B, k] { return launch_potrf_diag(B, k); }});
+    v.push_back({1u << 1, "potrf_panel", [B, k] { return launch_potrf_panel(B, k); }});
+    v.push_back({(1u << 1) | (k == 0 ? 1u << 2 : 0u), "potrf_update", [B, k] { return launch_potrf_update(B, k); }});
   }
   const int lv = trtri_levels(B.nblk_max);
   for (int l = 0; l < lv; ++l) {
-    hip_check(launch_trtri_level(B, 1 << l, 0), "trtri step 0");
-    hip_check(launch_trtri_level(B, 1 << l, 1), "trtri step 1");
+    v.push_back({1u << 3, "trtri step 0", [B, l] { return launch_trtri_level(B, 1 << l, 0); }});
+    v.push_back({1u << 3, "trtri step 1", [B, l] { return launch_trtri_level(B, 1 << l, 1); }});
   }
-  hip_check(launch_solve_fy(B), "solve_fy");
-  hip_check(launch_gls(B), "gls");
-  hip_check(launch_solve_e(B), "solve_e");
-  hip_check(launch_solve_x(B), "solve_x");
-  hip_check(launch_finalize_ll(B), "finalize_ll");
-  hip_check(launch_rcond_estimate(B), "rcond_estimate");
+  v.push_back({1u << 4, "solve_fy", [B] { return launch_solve_fy(B); }});
+  v.push_back({1u << 4, "gls", [B] { return launch_gls(B); }});
+  v.push_back({1u << 4, "solve_e", [B] { return launch_solve_e(B); }});
+  v.push_back({1u << 4, "solve_x", [B] { return launch_solve_x(B); }});
+  v.push_back({1u << 4, "finalize_ll", [B] { return launch_finalize_ll(B); }});
+  v.push_back({1u << 4, "rcond_estimate", [B] { return launch_rcond_estimate(B); }});
   if (want_grad) {
-    hip_check(launch_lauum_adjoint(B), "lauum_adjoint");
-    hip_check(launch_finalize_grad(B), "finalize_grad");
+    v.push_back({1u << 5, "lauum_adjoint", [B] { return launch_lauum_adjoint(B); }});
+    v.push_back({0u, "finalize_grad", [B] { return launch_finalize_grad(B); }});
   }
+  return v;
+}
+
+static void run_steps(const std::vector<Runtime::Step>& v, size_t a, size_t b) {
+  for (size_t i = a; i < b; ++i) hip_check(v[i].fn(), v[i].name);
+}
+
+// [first, last) range of the steps of `stage` (empty when it has none)
+static std::pair<size_t, size_t> stage_range(const std::vector<Runtime::Step>& v, int stage) {
+  size_t a = v.size(), b = v.size();
+  if (stage < 0) return {a, b};
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i].stages & (1u << stage)) {
+      if (a == v.size()) a = i;
+      b = i + 1;
+    }
+  if (a == v.size()) b = a;
+  return {a, b};
+}
+
+hipGraph_t Runtime::capture(const std::vector<Step>& v, size_t a, size_t b) {
+  hipGraph_t g = nullptr;
+  hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    run_steps(v, a, b);
+  } catch (...) {
+    hipStreamEndCapture(stream_, &g);
+    if (g) hipGraphDestroy(g);
+    throw;
+  }
+  hip_check(hipStreamEndCapture(stream_, &g), "end capture");
+  return g;
+}
+
+void Runtime::set_timed_stage(int stage) {
+  if (stage < -1 || stage > 5) throw std::invalid_argument("unknown stage id");
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  for (auto& e : tev_)
+    if (e == nullptr) hip_check(hipEventCreate(&e), "hipEventCreate");
+  timed_stage_ = stage;
+  timed_sum_ms_ = 0.0;
+  timed_count_ = 0;
+}
+
+void Runtime::take_stage_time(double* avg_ms, int* count) {
+  *avg_ms = timed_count_ > 0 ? timed_sum_ms_ / timed_count_ : 0.0;
+  *count = timed_count_;
+  timed_sum_ms_ = 0.0;
+  timed_count_ = 0;
 }
 
 void Runtime::run_pipeline(bool want_grad, BenchMap* bench) {
@@ -193,45 +249,78 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench) {
   B.stream = stream_;
   hip_check(hipMemcpyAsync(d_params_, h_params_, sizeof(SlotParams) * nslots_, hipMemcpyHostToDevice, stream_),
             "upload params");
+  const std::vector<Step> steps = pipeline_steps(B, want_grad);
+  const auto rng = stage_range(steps, timed_stage_);
+  const bool timed = rng.first < rng.second;
   if (bench != nullptr) {
-    // timed, uncaptured run: one event pair around the whole pipeline
+    // uncaptured run with one event pair around the whole pipeline
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "event");
     hip_check(hipEventCreate(&e1), "event");
     hip_check(hipEventRecord(e0, stream_), "event record");
-    launch_all(B, want_grad);
+    run_steps(steps, 0, steps.size());
     hip_check(hipEventRecord(e1, stream_), "event record");
     hip_check(hipEventSynchronize(e1), "event sync");
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "event elapsed");
     (*bench)["gpu pipeline"] += ms;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
   } else if (use_graph) {
     GraphEntry& G = graphs_[want_grad ? 1 : 0];
-    if (G.version != layout_version_ || G.exec == nullptr) {
+    if (G.version != layout_version_ || G.exec == nullptr || G.timed_stage != timed_stage_) {
       if (G.exec) hip_check(hipGraphExecDestroy(G.exec), "graph exec destroy");
       G.exec = nullptr;
-      hipGraph_t graph;
-      hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
-      try {
-        launch_all(B, want_grad);
-      } catch (...) {
-        hipStreamEndCapture(stream_, &graph);
-        throw;
+      hipGraph_t graph = nullptr;
+      if (!timed) {
+        graph = capture(steps, 0, steps.size());
+      } else {
+        // events captured from a stream are not timeable here, so the timed
+        // stage sits between explicit event-record nodes of a parent graph
+        // whose other nodes are the captured segments before and after it
+        hip_check(hipGraphCreate(&graph, 0), "graph create");
+        hipGraphNode_t prev = nullptr, node = nullptr;
+        auto add_child = [&](size_t a, size_t b) {
+          if (a >= b) return;
+          hipGraph_t child = capture(steps, a, b);
+          hip_check(hipGraphAddChildGraphNode(&node, graph, prev ? &prev : nullptr, prev ? 1 : 0, child),
+                    "add child graph");
+          hip_check(hipGraphDestroy(child), "graph destroy");
+          prev = node;
+        };
+        auto add_event = [&](hipEvent_t e) {
+          hip_check(hipGraphAddEventRecordNode(&node, graph, prev ? &prev : nullptr, prev ? 1 : 0, e),
+                    "add event node");
+          prev = node;
+        };
+        add_child(0, rng.first);
+        add_event(tev_[0]);
+        add_child(rng.first, rng.second);
+        add_event(tev_[1]);
+        add_child(rng.second, steps.size());
       }
-      hip_check(hipStreamEndCapture(stream_, &graph), "end capture");
       hip_check(hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0), "graph instantiate");
       hip_check(hipGraphDestroy(graph), "graph destroy");
       G.version = layout_version_;
+      G.timed_stage = timed_stage_;
     }
     hip_check(hipGraphLaunch(G.exec, stream_), "graph launch");
   } else {
-    launch_all(B, want_grad);
+    run_steps(steps, 0, rng.first);
+    if (timed) hip_check(hipEventRecord(tev_[0], stream_), "timing event");
+    run_steps(steps, rng.first, rng.second);
+    if (timed) hip_check(hipEventRecord(tev_[1], stream_), "timing event");
+    run_steps(steps, rng.second, steps.size());
   }
   hip_check(hipMemcpyAsync(h_out_, d_out_, sizeof(SlotOut) * nslots_, hipMemcpyDeviceToHost, stream_),
             "download results");
   hip_check(hipStreamSynchronize(stream_), "stream sync");
+  if (timed && bench == nullptr) {
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, tev_[0], tev_[1]), "stage timing");
+    timed_sum_ms_ += ms;
+    ++timed_count_;
+  }
 }
 
 void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std::vector<EvalResult>& res,
@@ -262,7 +351,8 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
     Q.num_nugget = chol.num_nugget;
     Q.min_rcond = chol.min_rcond;
     Q.inc = 0;
-    Q.active = 1;
+    Q.active = R.active ? 1 : 0;
+    if (!R.active) res[s].ll = std::numeric_limits<double>::quiet_NaN();
     Q.est_sigma2 = R.flags.est_sigma2;
     Q.est_nugget = R.flags.est_nugget;
     Q.rcond_check = chol.rcond_check;
@@ -351,6 +441,13 @@ void Runtime::read_slot(int s, int what, double* out) {
     case 5:
       copy_cols(v.V + (size_t)(v.p + 3) * ld, 1);
       break;
+    case 6: {  // R* (p x p, upper), stored by the GLS kernel
+      const Slot& S = h_slots_[s];
+      if (v.p > 0)
+        hip_check(hipMemcpy(out, S.red + red_rstar_off(S.nblk), sizeof(double) * v.p * v.p, hipMemcpyDeviceToHost),
+                  "read R*");
+      break;
+    }
     default:
       throw std::invalid_argument("unknown buffer id");
   }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -44,6 +45,9 @@ struct EvalRequest {
   const double* gamma = nullptr;  // d or d+1 entries
   int gamma_n = 0;
   ModelFlags flags;
+  // an inactive request keeps its slot (same layout, same captured graph)
+  // but is not evaluated; its result is left with ll = NaN
+  bool active = true;
 };
 
 enum EvalStatus { EVAL_OK = 0, EVAL_CHOL_FAILED = 1, EVAL_NUGGET_NOT_POSITIVE = 2 };
@@ -100,13 +104,29 @@ class Runtime {
 
   CholConfig chol;
   bool use_graph = true;
+
+  // Live timing of one pipeline stage with a HIP event pair recorded on the
+  // pipeline stream (inside the captured graph when graphs are used).
+  // Stage ids as LKGPU_STAGE_* in include/lkgpu.h; -1 disables.
+  void set_timed_stage(int stage);
+  void take_stage_time(double* avg_ms, int* count);
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }
 
  private:
   void ensure_slots(const std::vector<EvalRequest>& reqs);
   void run_pipeline(bool want_grad, BenchMap* bench);
-  void launch_all(const Batch& B, bool want_grad);
+
+ public:
+  struct Step {
+    unsigned stages;  // bit s set when the launch belongs to stage s
+    const char* name;
+    std::function<hipError_t()> fn;
+  };
+
+ private:
+  std::vector<Step> pipeline_steps(const Batch& B, bool want_grad) const;
+  hipGraph_t capture(const std::vector<Step>& v, size_t a, size_t b);
 
   int device_ = 0;
   hipStream_t stream_ = nullptr;
@@ -131,9 +151,14 @@ class Runtime {
   // graph cache: (layout_version, want_grad) -> exec
   struct GraphEntry {
     uint64_t version = ~0ull;
+    int timed_stage = -1;
     hipGraphExec_t exec = nullptr;
   };
   GraphEntry graphs_[2];
+  int timed_stage_ = -1;
+  hipEvent_t tev_[2] = {nullptr, nullptr};
+  double timed_sum_ms_ = 0.0;
+  int timed_count_ = 0;
 };
 
 }  // namespace lk
