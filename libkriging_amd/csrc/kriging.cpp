@@ -260,83 +260,8 @@ double Kriging::logLikelihood() {
 // ---------------------------------------------------------------------------
 void Kriging::fit(const double* y, const double* X, int n, int d, RegModel regmodel, bool normalize,
                   const std::string& optim, const std::string& objective, const Parameters& P) {
-  if (objective != "LL")
-    throw std::invalid_argument("Unsupported objective on this path: " + objective + " (LL)");
-  optim_ = optim;
-  objective_ = objective;
-  setup(y, X, n, d, regmodel, normalize);
-
-  // beta fixed or estimated (KrigingImpl.cpp:816-826)
-  est_beta_ = P.is_beta_estim && regmodel != RegModel::None;
-  if (!est_beta_ && P.beta.has_value() && !P.beta->empty()) {
-    est_beta_ = false;
-    beta_ = *P.beta;
-    if ((int)beta_.size() != p_) throw std::invalid_argument("beta must have length " + std::to_string(p_));
-    if (normalize_)
-      for (double& b : beta_) b /= scaleY_;
-  } else {
-    est_beta_ = true;
-  }
-  // theta0 rows (normalised)
-  std::vector<double> theta0;
-  int rows0 = 0;
-  if (P.theta.has_value()) {
-    theta0 = *P.theta;
-    rows0 = P.theta_rows;
-    if (rows0 <= 0 || (int)theta0.size() != rows0 * d)
-      throw std::runtime_error("Dimension of theta should be nx" + std::to_string(d));
-    if (normalize_)
-      for (int r = 0; r < rows0; ++r)
-        for (int k = 0; k < d; ++k) theta0[(size_t)r * d + k] /= scaleX_[k];
-  }
-
-  if (optim == "none") {
-    if (!P.theta.has_value())
-      throw std::runtime_error("Theta should be given (1x" + std::to_string(d) + ") matrix, when optim=none");
-    std::vector<double> th(theta0.begin(), theta0.begin() + d);
-    est_theta_ = false;
-    double sigma2 = -1;
-    est_sigma2_ = P.is_sigma2_estim;
-    if (P.sigma2.has_value()) {
-      sigma2 = *P.sigma2;
-      if (normalize_) sigma2 /= scaleY_ * scaleY_;
-    } else {
-      est_sigma2_ = true;
-    }
-    double alpha = 1.0, nugget_param = 0.0;
-    if (noise_ == NoiseModel::Nugget) {
-      est_nugget_ = P.is_nugget_estim;
-      if (P.nugget.has_value()) {
-        nugget_param = *P.nugget;
-        if (normalize_) nugget_param /= scaleY_ * scaleY_;
-      }
-      alpha = (sigma2 > 0 && (sigma2 + nugget_param) > 0) ? sigma2 / (sigma2 + nugget_param)
-                                                           : 1.0 - NUGGET_ALPHA_LOWER;
-      sigma2_ = sigma2;
-      nugget_ = nugget_param;
-    } else if (!est_sigma2_) {
-      sigma2_ = sigma2;
-    }
-    alpha_ = alpha;
-    commit(th, alpha);
-    return;
-  }
-  if (optim.rfind("BFGS", 0) != 0)
-    throw std::runtime_error("Unsupported optim: " + optim + " (supported are: none, BFGS[#])");
-  // Optim::parse_method("BFGS64") -> 64 starts
-  int multistart = 1;
-  {
-    const std::string rest = optim.substr(4);
-    size_t k = 0;
-    while (k < rest.size() && std::isdigit((unsigned char)rest[k])) ++k;
-    if (k > 0) multistart = std::max(1, std::atoi(rest.substr(0, k).c_str()));
-  }
-  Parameters Pn = P;
-  if (P.theta.has_value()) {
-    Pn.theta = theta0;
-    Pn.theta_rows = rows0;
-  }
-  fit_bfgs(multistart, Pn);
+  std::unique_ptr<FitJob> job = begin_fit(y, X, n, d, regmodel, normalize, optim, objective, P);
+  run_fit_jobs({job.get()});
 }
 
 namespace {
@@ -376,149 +301,57 @@ struct StartWorker {
 };
 }  // namespace
 
-void Kriging::fit_bfgs(int multistart, const Parameters& P) {
-  const OptimConfig& oc = optim_config();
-  const int n = n_, d = d_;
-  const bool nug = noise_ == NoiseModel::Nugget;
-  const int gd = d + (nug ? 1 : 0);
+// State of one fit between begin_fit and the commit: the multistart
+// workers of a BFGS fit (none for optim = "none") and the committed point.
+struct FitState : FitJob {
+  Kriging* m = nullptr;
+  bool bfgs = false;
+  int d = 0, gd = 0;
+  bool nug = false;
+  Reparam rp{0, false, true};
   std::vector<double> th_lo, th_up;
-  theta_bounds(th_lo, th_up);
+  std::vector<StartWorker> W;
+  ModelFlags flags;
+  // per-slot (theta, alpha) handed to the runtime for the current batch
+  std::vector<std::vector<double>> th_alpha;
+  // result: the point to commit
+  std::vector<double> theta;
+  double alpha = 1.0;
 
-  // start points (Kriging.cpp:1708-1765): Random::init(), then
-  // theta_lower + randu(multistart, d) % (upper - lower), column-major fill
-  g_engine.seed(g_seed);
-  std::uniform_real_distribution<double> unif{};
-  std::vector<double> rand0((size_t)multistart * d);  // row-major rows x d
-  for (int k = 0; k < d; ++k)
-    for (int r = 0; r < multistart; ++r) rand0[(size_t)r * d + k] = th_lo[k] + unif(g_engine) * (th_up[k] - th_lo[k]);
-  std::vector<double> theta0;
-  if (P.theta.has_value()) {
-    multistart = std::max(multistart, P.theta_rows);
-    theta0 = *P.theta;
-    theta0.insert(theta0.end(), rand0.begin(), rand0.end());
-    theta0.resize((size_t)multistart * d);
-  } else {
-    theta0 = rand0;
-  }
-  // extra0 (alpha) for the nugget model (Kriging.cpp:1774-1783)
-  std::vector<double> extra0;
-  double ext_lo = 0.0, ext_up = 1.0;
-  if (nug) {
-    ext_lo = NUGGET_ALPHA_LOWER;
-    ext_up = 1.0;
-    if (P.sigma2.has_value() && P.nugget.has_value()) {
-      const double s = *P.sigma2, nu = *P.nugget;
-      extra0.push_back((s > 0 && (s + nu) > 0) ? s / (s + nu) : ext_lo + (ext_up - ext_lo) * 0.5);
-    } else {
-      const int rows = multistart;
-      for (int r = 0; r < rows; ++r) {
-        const double u = unif(g_engine);
-        extra0.push_back(ext_lo + (ext_up - ext_lo) * (1.0 - u * u * u));
-      }
-    }
-  }
-  Reparam rp{d, nug, oc.reparametrize};
-  std::vector<double> glo(gd), gup(gd);
-  {
-    rp.to_gamma(th_lo.data(), ext_lo, glo.data());
-    rp.to_gamma(th_up.data(), ext_up, gup.data());
-  }
-  // estimation flags (Kriging.cpp:1832-1849)
-  est_sigma2_ = P.is_sigma2_estim;
-  if (!est_sigma2_ && P.sigma2.has_value()) {
-    sigma2_ = *P.sigma2;
-    if (normalize_) sigma2_ /= scaleY_ * scaleY_;
-  } else {
-    est_sigma2_ = true;
-  }
-  if (nug) {
-    est_nugget_ = P.is_nugget_estim;
-    if (!est_nugget_ && P.nugget.has_value()) {
-      nugget_ = *P.nugget;
-      if (normalize_) nugget_ /= scaleY_ * scaleY_;
-    } else {
-      est_nugget_ = true;
-    }
-  }
-  const double pgtol = oc.gradient_tolerance;
-  const double factr = oc.objective_rel_tolerance / 1e-13;
+  int nslots() const { return bfgs ? (int)W.size() : 0; }
 
-  std::vector<StartWorker> W(multistart);
-  auto restart_point = [&](StartWorker& w, const std::vector<double>& th) {
-    w.gamma.assign(gd, 0.0);
-    rp.to_gamma(th.data(), nug ? w.extra0 : 1.0, w.gamma.data());
-    for (int k = 0; k < gd; ++k) {
-      w.glo[k] = std::min(w.gamma[k], w.glo[k]);
-      w.gup[k] = std::max(w.gamma[k], w.gup[k]);
-    }
-    w.opt = std::make_unique<LBFGSB>(gd, oc.lbfgs_memory);
-    w.opt->max_iter = oc.max_iteration;
-    w.opt->pgtol = pgtol;
-    w.opt->factr = factr;
-    w.opt->start(w.gamma, w.glo, w.gup);
-  };
-  for (int s = 0; s < multistart; ++s) {
-    StartWorker& w = W[s];
-    w.idx = s;
-    w.theta_start.assign(theta0.begin() + (size_t)s * d, theta0.begin() + (size_t)(s + 1) * d);
-    w.extra0 = nug ? extra0[s % extra0.size()] : 1.0;
-    w.glo = glo;
-    w.gup = gup;
-    restart_point(w, w.theta_start);
-    w.best_gamma = w.gamma;
-  }
-
-  // objective of one start: -LL in gamma, gradient -dLL/dgamma
-  const ModelFlags flags = flags_for_eval();
-  fit_evals_ = 0;
-  fit_batches_ = 0;
-  std::vector<EvalRequest> reqs(multistart);
-  std::vector<std::vector<double>> th_alpha(multistart, std::vector<double>(gd));
-  std::vector<EvalResult> res;
-  auto eval_pending = [&](bool want_grad, auto&& point_of) {
-    // one batched pipeline over every start that waits for an evaluation;
-    // the others stay in their slots, inactive, so the layout (and the
-    // captured graph) does not change between iterations
-    for (int s = 0; s < multistart; ++s) {
-      const double* g = point_of(W[s]);
-      reqs[s].problem = problem_;
+  // fill this job's requests; returns how many are active
+  int requests(EvalRequest* reqs, bool final_pass) {
+    int nact = 0;
+    for (int s = 0; s < (int)W.size(); ++s) {
+      StartWorker& w = W[s];
+      const double* g = nullptr;
+      if (final_pass)
+        g = w.success ? w.best_gamma.data() : nullptr;
+      else if (!w.done && w.opt->need_eval())
+        g = w.opt->x().data();
+      reqs[s].problem = m->problem_;
       reqs[s].gamma_n = gd;
       reqs[s].flags = flags;
       reqs[s].active = g != nullptr;
       if (g != nullptr) {
-        double alpha = 1.0;
-        rp.from_gamma(g, th_alpha[s].data(), &alpha);
-        if (nug) th_alpha[s][d] = alpha;
+        double a = 1.0;
+        rp.from_gamma(g, th_alpha[s].data(), &a);
+        if (nug) th_alpha[s][d] = a;
+        ++nact;
       }
       reqs[s].gamma = th_alpha[s].data();
     }
-    if (batched_fit_) {
-      rt_->evaluate(reqs, want_grad, res);
-      ++fit_batches_;
-    } else {
-      res.assign(multistart, EvalResult{});
-      std::vector<EvalRequest> one(1);
-      std::vector<EvalResult> r1;
-      for (int s = 0; s < multistart; ++s) {
-        if (!reqs[s].active) continue;
-        one[0] = reqs[s];
-        rt_->evaluate(one, want_grad, r1);
-        res[s] = r1[0];
-        ++fit_batches_;
-      }
-    }
-    for (int s = 0; s < multistart; ++s) fit_evals_ += reqs[s].active ? 1 : 0;
-  };
+    m->fit_evals_ += nact;
+    return nact;
+  }
 
-  std::vector<double> gbuf(gd);
-  for (;;) {
-    bool any = false;
-    for (auto& w : W) any = any || (!w.done && w.opt->need_eval());
-    if (!any) break;
-    eval_pending(true, [](StartWorker& w) -> const double* {
-      return (!w.done && w.opt->need_eval()) ? w.opt->x().data() : nullptr;
-    });
-    for (int s = 0; s < multistart; ++s) {
+  // feed one batch of results to the optimisers; restart rules of
+  // Kriging.cpp:1994-2041 when a run ends
+  void consume(const EvalRequest* reqs, const EvalResult* res) {
+    const OptimConfig& oc = optim_config();
+    std::vector<double> gbuf(gd), th(d), theta_part(d);
+    for (int s = 0; s < (int)W.size(); ++s) {
       StartWorker& w = W[s];
       if (!reqs[s].active) continue;
       if (res[s].status != EVAL_OK) {
@@ -530,20 +363,17 @@ void Kriging::fit_bfgs(int multistart, const Parameters& P) {
         continue;
       }
       const double f = -res[s].ll;
-      double alpha = 1.0;
-      std::vector<double> th(d);
-      rp.from_gamma(w.opt->x().data(), th.data(), &alpha);
-      rp.to_gamma_grad(th.data(), alpha, res[s].grad, gbuf.data());
+      double a = 1.0;
+      rp.from_gamma(w.opt->x().data(), th.data(), &a);
+      rp.to_gamma_grad(th.data(), a, res[s].grad, gbuf.data());
       w.opt->tell(f, gbuf);
       if (!w.opt->done()) continue;
-      // one optimiser run finished: restart rules (Kriging.cpp:1994-2041)
       w.gamma = w.opt->x_opt();
       const double fopt = w.opt->f_opt();
       if (fopt < w.best_f) {
         w.best_f = fopt;
         w.best_gamma = w.gamma;
       }
-      std::vector<double> theta_part(d);
       double a_tmp;
       rp.from_gamma(w.gamma.data(), theta_part.data(), &a_tmp);
       double sol_to_lb = std::numeric_limits<double>::infinity();
@@ -563,31 +393,277 @@ void Kriging::fit_bfgs(int multistart, const Parameters& P) {
       }
     }
   }
-  // final objective at each start's best point (fit_ofn(best_gamma, nullptr))
-  eval_pending(false, [](StartWorker& w) -> const double* { return w.success ? w.best_gamma.data() : nullptr; });
-  int best = -1, nsucc = 0;
-  double min_ofn = std::numeric_limits<double>::infinity();
-  for (int s = 0; s < multistart; ++s) {
-    StartWorker& w = W[s];
-    if (!w.success) continue;
-    if (res[s].status != EVAL_OK) {
-      w.success = false;
-      continue;
+
+  // best start from the final objective values (Kriging.cpp:2097-2119)
+  void select_best(const EvalResult* res) {
+    int best = -1, nsucc = 0;
+    double min_ofn = std::numeric_limits<double>::infinity();
+    for (int s = 0; s < (int)W.size(); ++s) {
+      StartWorker& w = W[s];
+      if (!w.success) continue;
+      if (res[s].status != EVAL_OK) {
+        w.success = false;
+        continue;
+      }
+      w.objective = -res[s].ll;
+      ++nsucc;
+      if (w.objective < min_ofn) {
+        min_ofn = w.objective;
+        best = s;
+      }
     }
-    w.objective = -res[s].ll;
-    ++nsucc;
-    if (w.objective < min_ofn) {
-      min_ofn = w.objective;
-      best = s;
+    if (nsucc == 0) throw std::runtime_error("All " + std::to_string(W.size()) + " optimization attempts failed");
+    theta.assign(d, 0.0);
+    alpha = 1.0;
+    rp.from_gamma(W[best].best_gamma.data(), theta.data(), &alpha);
+    m->est_theta_ = true;
+    m->alpha_ = alpha;
+  }
+
+  void restart_point(StartWorker& w, const std::vector<double>& th) {
+    const OptimConfig& oc = optim_config();
+    w.gamma.assign(gd, 0.0);
+    rp.to_gamma(th.data(), nug ? w.extra0 : 1.0, w.gamma.data());
+    for (int k = 0; k < gd; ++k) {
+      w.glo[k] = std::min(w.gamma[k], w.glo[k]);
+      w.gup[k] = std::max(w.gamma[k], w.gup[k]);
+    }
+    w.opt = std::make_unique<LBFGSB>(gd, oc.lbfgs_memory);
+    w.opt->max_iter = oc.max_iteration;
+    w.opt->pgtol = oc.gradient_tolerance;
+    w.opt->factr = oc.objective_rel_tolerance / 1e-13;
+    w.opt->start(w.gamma, w.glo, w.gup);
+  }
+};
+
+std::unique_ptr<FitJob> Kriging::begin_fit(const double* y, const double* X, int n, int d, RegModel regmodel,
+                                           bool normalize, const std::string& optim, const std::string& objective,
+                                           const Parameters& P) {
+  if (objective != "LL")
+    throw std::invalid_argument("Unsupported objective on this path: " + objective + " (LL)");
+  int multistart = 1;
+  const bool none = optim == "none";
+  if (!none) {
+    if (optim.rfind("BFGS", 0) != 0)
+      throw std::runtime_error("Unsupported optim: " + optim + " (supported are: none, BFGS[#])");
+    // Optim::parse_method("BFGS64") -> 64 starts
+    const std::string rest = optim.substr(4);
+    size_t k = 0;
+    while (k < rest.size() && std::isdigit((unsigned char)rest[k])) ++k;
+    if (k > 0) multistart = std::max(1, std::atoi(rest.substr(0, k).c_str()));
+  }
+  optim_ = optim;
+  objective_ = objective;
+  setup(y, X, n, d, regmodel, normalize);
+  fit_evals_ = 0;
+  fit_batches_ = 0;
+
+  // beta fixed or estimated (KrigingImpl.cpp:816-826)
+  est_beta_ = P.is_beta_estim && regmodel != RegModel::None;
+  if (!est_beta_ && P.beta.has_value() && !P.beta->empty()) {
+    est_beta_ = false;
+    beta_ = *P.beta;
+    if ((int)beta_.size() != p_) throw std::invalid_argument("beta must have length " + std::to_string(p_));
+    if (normalize_)
+      for (double& b : beta_) b /= scaleY_;
+  } else {
+    est_beta_ = true;
+  }
+  // theta0 rows (normalised)
+  std::vector<double> theta0;
+  int rows0 = 0;
+  if (P.theta.has_value()) {
+    theta0 = *P.theta;
+    rows0 = P.theta_rows;
+    if (rows0 <= 0 || (int)theta0.size() != rows0 * d)
+      throw std::runtime_error("Dimension of theta should be nx" + std::to_string(d));
+    if (normalize_)
+      for (int r = 0; r < rows0; ++r)
+        for (int k = 0; k < d; ++k) theta0[(size_t)r * d + k] /= scaleX_[k];
+  }
+
+  auto st = std::make_unique<FitState>();
+  st->m = this;
+  st->d = d;
+  st->nug = noise_ == NoiseModel::Nugget;
+  st->gd = d + (st->nug ? 1 : 0);
+
+  if (none) {
+    if (!P.theta.has_value())
+      throw std::runtime_error("Theta should be given (1x" + std::to_string(d) + ") matrix, when optim=none");
+    std::vector<double> th(theta0.begin(), theta0.begin() + d);
+    est_theta_ = false;
+    double sigma2 = -1;
+    est_sigma2_ = P.is_sigma2_estim;
+    if (P.sigma2.has_value()) {
+      sigma2 = *P.sigma2;
+      if (normalize_) sigma2 /= scaleY_ * scaleY_;
+    } else {
+      est_sigma2_ = true;
+    }
+    double alpha = 1.0, nugget_param = 0.0;
+    if (noise_ == NoiseModel::Nugget) {
+      est_nugget_ = P.is_nugget_estim;
+      if (P.nugget.has_value()) {
+        nugget_param = *P.nugget;
+        if (normalize_) nugget_param /= scaleY_ * scaleY_;
+      }
+      alpha = (sigma2 > 0 && (sigma2 + nugget_param) > 0) ? sigma2 / (sigma2 + nugget_param)
+                                                           : 1.0 - NUGGET_ALPHA_LOWER;
+      sigma2_ = sigma2;
+      nugget_ = nugget_param;
+    } else if (!est_sigma2_) {
+      sigma2_ = sigma2;
+    }
+    alpha_ = alpha;
+    st->theta = th;
+    st->alpha = alpha;
+    return st;
+  }
+
+  // ---- BFGS multistart (Kriging.cpp:1591-2120) ----
+  const OptimConfig& oc = optim_config();
+  const bool nug = st->nug;
+  const int gd = st->gd;
+  st->bfgs = true;
+  theta_bounds(st->th_lo, st->th_up);
+  const std::vector<double>& th_lo = st->th_lo;
+  const std::vector<double>& th_up = st->th_up;
+
+  // start points (Kriging.cpp:1708-1765): Random::init(), then
+  // theta_lower + randu(multistart, d) % (upper - lower), column-major fill
+  g_engine.seed(g_seed);
+  std::uniform_real_distribution<double> unif{};
+  std::vector<double> rand0((size_t)multistart * d);  // row-major rows x d
+  for (int k = 0; k < d; ++k)
+    for (int r = 0; r < multistart; ++r) rand0[(size_t)r * d + k] = th_lo[k] + unif(g_engine) * (th_up[k] - th_lo[k]);
+  std::vector<double> thetas;
+  if (P.theta.has_value()) {
+    multistart = std::max(multistart, rows0);
+    thetas = theta0;
+    thetas.insert(thetas.end(), rand0.begin(), rand0.end());
+    thetas.resize((size_t)multistart * d);
+  } else {
+    thetas = rand0;
+  }
+  // extra0 (alpha) for the nugget model (Kriging.cpp:1774-1783)
+  std::vector<double> extra0;
+  double ext_lo = 0.0, ext_up = 1.0;
+  if (nug) {
+    ext_lo = NUGGET_ALPHA_LOWER;
+    ext_up = 1.0;
+    if (P.sigma2.has_value() && P.nugget.has_value()) {
+      const double s = *P.sigma2, nu = *P.nugget;
+      extra0.push_back((s > 0 && (s + nu) > 0) ? s / (s + nu) : ext_lo + (ext_up - ext_lo) * 0.5);
+    } else {
+      for (int r = 0; r < multistart; ++r) {
+        const double u = unif(g_engine);
+        extra0.push_back(ext_lo + (ext_up - ext_lo) * (1.0 - u * u * u));
+      }
     }
   }
-  if (nsucc == 0) throw std::runtime_error("All " + std::to_string(multistart) + " optimization attempts failed");
-  std::vector<double> th(d);
-  double alpha = 1.0;
-  rp.from_gamma(W[best].best_gamma.data(), th.data(), &alpha);
-  est_theta_ = true;
-  alpha_ = alpha;
-  commit(th, alpha);
+  st->rp = Reparam{d, nug, oc.reparametrize};
+  std::vector<double> glo(gd), gup(gd);
+  st->rp.to_gamma(th_lo.data(), ext_lo, glo.data());
+  st->rp.to_gamma(th_up.data(), ext_up, gup.data());
+  // estimation flags (Kriging.cpp:1832-1849)
+  est_sigma2_ = P.is_sigma2_estim;
+  if (!est_sigma2_ && P.sigma2.has_value()) {
+    sigma2_ = *P.sigma2;
+    if (normalize_) sigma2_ /= scaleY_ * scaleY_;
+  } else {
+    est_sigma2_ = true;
+  }
+  if (nug) {
+    est_nugget_ = P.is_nugget_estim;
+    if (!est_nugget_ && P.nugget.has_value()) {
+      nugget_ = *P.nugget;
+      if (normalize_) nugget_ /= scaleY_ * scaleY_;
+    } else {
+      est_nugget_ = true;
+    }
+  }
+  st->W.resize(multistart);
+  st->th_alpha.assign(multistart, std::vector<double>(gd));
+  for (int s = 0; s < multistart; ++s) {
+    StartWorker& w = st->W[s];
+    w.idx = s;
+    w.theta_start.assign(thetas.begin() + (size_t)s * d, thetas.begin() + (size_t)(s + 1) * d);
+    w.extra0 = nug ? extra0[s % extra0.size()] : 1.0;
+    w.glo = glo;
+    w.gup = gup;
+    st->restart_point(w, w.theta_start);
+    w.best_gamma = w.gamma;
+  }
+  st->flags = flags_for_eval();
+  return st;
+}
+
+void Kriging::run_fit_jobs(const std::vector<FitJob*>& jobs_in) {
+  std::vector<FitState*> jobs;
+  for (FitJob* j : jobs_in) {
+    FitState* f = dynamic_cast<FitState*>(j);
+    if (f == nullptr) throw std::invalid_argument("run_fit_jobs: not a job of begin_fit");
+    jobs.push_back(f);
+  }
+  if (jobs.empty()) return;
+  Runtime* rt = jobs[0]->m->rt_;
+  for (FitState* f : jobs)
+    if (f->m->rt_ != rt) throw std::invalid_argument("run_fit_jobs: all models must use the same device");
+  // every job's starts keep fixed slots for the whole optimisation (inactive
+  // ones stay in place), so the pool layout and the captured graph are
+  // reused from one step to the next
+  std::vector<int> base(jobs.size() + 1, 0);
+  for (size_t j = 0; j < jobs.size(); ++j) base[j + 1] = base[j] + jobs[j]->nslots();
+  const int total = base.back();
+  const bool batched = std::all_of(jobs.begin(), jobs.end(), [](FitState* f) { return f->m->batched_fit_; });
+  std::vector<EvalRequest> reqs(total);
+  std::vector<EvalResult> res;
+  auto evaluate = [&](bool want_grad) {
+    if (batched) {
+      rt->evaluate(reqs, want_grad, res);
+      for (FitState* f : jobs)
+        if (f->bfgs) f->m->fit_batches_++;
+      return;
+    }
+    res.assign(total, EvalResult{});
+    std::vector<EvalRequest> one(1);
+    std::vector<EvalResult> r1;
+    for (size_t j = 0; j < jobs.size(); ++j)
+      for (int s = base[j]; s < base[j + 1]; ++s) {
+        if (!reqs[s].active) continue;
+        one[0] = reqs[s];
+        rt->evaluate(one, want_grad, r1);
+        res[s] = r1[0];
+        jobs[j]->m->fit_batches_++;
+      }
+  };
+  if (total > 0) {
+    for (;;) {
+      int nact = 0;
+      for (size_t j = 0; j < jobs.size(); ++j) nact += jobs[j]->requests(reqs.data() + base[j], false);
+      if (nact == 0) break;
+      evaluate(true);
+      for (size_t j = 0; j < jobs.size(); ++j) jobs[j]->consume(reqs.data() + base[j], res.data() + base[j]);
+    }
+    // final objective at each start's best point (fit_ofn(best_gamma, nullptr))
+    for (size_t j = 0; j < jobs.size(); ++j) jobs[j]->requests(reqs.data() + base[j], true);
+    evaluate(false);
+    for (size_t j = 0; j < jobs.size(); ++j)
+      if (jobs[j]->bfgs) jobs[j]->select_best(res.data() + base[j]);
+  }
+  // commit every model in one batch, one slot each
+  std::vector<std::vector<double>> gam(jobs.size());
+  std::vector<EvalRequest> creq(jobs.size());
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    gam[j] = jobs[j]->theta;
+    if (jobs[j]->nug) gam[j].push_back(jobs[j]->alpha);
+    creq[j] = jobs[j]->m->commit_request(gam[j]);
+    creq[j].flags.alpha = jobs[j]->alpha;
+  }
+  std::vector<EvalResult> cres;
+  rt->evaluate(creq, false, cres);
+  for (size_t j = 0; j < jobs.size(); ++j) jobs[j]->m->commit_apply(jobs[j]->theta, jobs[j]->alpha, (int)j, cres[j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -666,23 +742,30 @@ void Kriging::predict(const double* Xn, int m, bool want_stdev, double* mean, do
 // Evaluate the model at (theta, alpha), keep L^-1, z, M, R* on the device
 // for predict, and set sigma2 / nugget / beta as the reference's commit
 // (Kriging.cpp:2156-2202 and the optim="none" branch 1681-1700).
-void Kriging::commit(const std::vector<double>& theta, double alpha) {
-  const int d = d_, p = p_;
-  std::vector<double> g(theta);
-  if (noise_ == NoiseModel::Nugget) g.push_back(alpha);
+EvalRequest Kriging::commit_request(const std::vector<double>& gamma) const {
   EvalRequest R;
   R.problem = problem_;
-  R.gamma = g.data();
-  R.gamma_n = (int)g.size();
+  R.gamma = gamma.data();
+  R.gamma_n = (int)gamma.size();
   R.flags = flags_for_eval();
-  R.flags.alpha = alpha;
-  std::vector<EvalRequest> reqs{R};
+  return R;
+}
+
+void Kriging::commit(const std::vector<double>& theta, double alpha) {
+  std::vector<double> g(theta);
+  if (noise_ == NoiseModel::Nugget) g.push_back(alpha);
+  std::vector<EvalRequest> reqs{commit_request(g)};
+  reqs[0].flags.alpha = alpha;
   std::vector<EvalResult> res;
   rt_->evaluate(reqs, false, res);
-  if (res[0].status != EVAL_OK)
-    throw std::runtime_error("[ERROR] Exceed max numerical nugget added to force chol matrix");
+  commit_apply(theta, alpha, 0, res[0]);
+}
+
+void Kriging::commit_apply(const std::vector<double>& theta, double alpha, int slot, const EvalResult& r) {
+  const int p = p_;
+  if (r.status != EVAL_OK) throw std::runtime_error("[ERROR] Exceed max numerical nugget added to force chol matrix");
   theta_ = theta;
-  const Runtime::SlotView v = rt_->slot_view(0);
+  const Runtime::SlotView v = rt_->slot_view(slot);
   npad_ = v.npad;
   hip_check(hipSetDevice(device_), "hipSetDevice");
   if (d_Linv_) hipFree(d_Linv_);
@@ -697,8 +780,8 @@ void Kriging::commit(const std::vector<double>& theta, double alpha) {
   if (p > 0)
     hip_check(hipMemcpyAsync(d_vec_, v.V, sizeof(double) * (size_t)npad_ * p, hipMemcpyDeviceToDevice, rt_->stream()),
               "copy model");
-  const double sse = res[0].sse;
-  std::vector<double> bhat(res[0].beta, res[0].beta + p);
+  const double sse = r.sse;
+  std::vector<double> bhat(r.beta, r.beta + p);
   if (est_beta_) {
     beta_ = bhat;
     hip_check(hipMemcpyAsync(d_vec_ + (size_t)npad_ * p, v.V + (size_t)(p + 2) * npad_, sizeof(double) * npad_,
@@ -707,8 +790,8 @@ void Kriging::commit(const std::vector<double>& theta, double alpha) {
   } else {
     // z = y* - M beta (Kriging.cpp:2170-2172)
     std::vector<double> ystar(n_), M((size_t)n_ * std::max(p, 1)), z(npad_, 0.0);
-    rt_->read_slot(0, 3, ystar.data());
-    if (p > 0) rt_->read_slot(0, 2, M.data());
+    rt_->read_slot(slot, 3, ystar.data());
+    if (p > 0) rt_->read_slot(slot, 2, M.data());
     for (int i = 0; i < n_; ++i) {
       double s = ystar[i];
       for (int a = 0; a < p; ++a) s -= M[i + (size_t)a * n_] * beta_[a];
@@ -719,7 +802,7 @@ void Kriging::commit(const std::vector<double>& theta, double alpha) {
               "upload z");
   }
   Rstar_.assign((size_t)p * p, 0.0);
-  if (p > 0) rt_->read_slot(0, 6, Rstar_.data());
+  if (p > 0) rt_->read_slot(slot, 6, Rstar_.data());
   hip_check(hipStreamSynchronize(rt_->stream()), "sync");
   const int n = n_;
   const double total_var = sse / n;
@@ -747,7 +830,6 @@ void Kriging::commit(const std::vector<double>& theta, double alpha) {
     sigma2_ = total_var;
   }
   is_empty_ = false;
-  (void)d;
 }
 
 }  // namespace lk

@@ -58,6 +58,15 @@ void random_reset_seed(unsigned seed);
 Runtime& runtime_for(int device);
 int default_device();
 
+// The optimisation part of one model's fit, run step by step: the pending
+// evaluations of every job handed to Kriging::run_fit_jobs go to the device
+// as one batched pipeline per optimiser step (several models' starts, e.g.
+// the groups of a NestedKriging, side by side).
+class FitJob {
+ public:
+  virtual ~FitJob() = default;
+};
+
 class Kriging {
  public:
   Kriging(const std::string& kernel, NoiseModel noise = NoiseModel::None);
@@ -69,6 +78,12 @@ class Kriging {
   void fit(const double* y, const double* X, int n, int d, RegModel regmodel = RegModel::Constant,
            bool normalize = false, const std::string& optim = "BFGS", const std::string& objective = "LL",
            const Parameters& parameters = Parameters{});
+  // fit() in two halves: set up data and start points, then optimise and
+  // commit every job together (all models on the same device)
+  std::unique_ptr<FitJob> begin_fit(const double* y, const double* X, int n, int d, RegModel regmodel,
+                                    bool normalize, const std::string& optim, const std::string& objective,
+                                    const Parameters& parameters);
+  static void run_fit_jobs(const std::vector<FitJob*>& jobs);
 
   // gamma = theta (d) or [theta, alpha] (d + 1, nugget model); returns LL
   double logLikelihoodFun(const double* gamma, int gamma_n, bool want_grad, double* grad);
@@ -106,13 +121,20 @@ class Kriging {
   int fit_batches() const { return fit_batches_; }
   // force the plain one-evaluation-per-launch path in fit (comparison only)
   void set_batched_fit(bool b) { batched_fit_ = b; }
+  // the uploaded design (device problem id) and the device runtime
+  int problem_id() const { return problem_; }
+  Runtime* runtime() const { return rt_; }
 
  private:
+  friend struct FitState;
   void setup(const double* y, const double* X, int n, int d, RegModel regmodel, bool normalize);
   ModelFlags flags_for_eval() const;
+  // commit in two halves so that many models commit in one batch: the
+  // request, then the copy of slot `slot`'s factors once it is evaluated
+  EvalRequest commit_request(const std::vector<double>& gamma) const;
+  void commit_apply(const std::vector<double>& theta, double alpha, int slot, const EvalResult& r);
   void commit(const std::vector<double>& theta, double alpha);
   void theta_bounds(std::vector<double>& lo, std::vector<double>& up) const;
-  void fit_bfgs(int multistart, const Parameters& parameters);
 
   std::string kernel_;
   NoiseModel noise_;

@@ -1,0 +1,257 @@
+// Device helpers shared by the likelihood kernels and the NestedKriging
+// kernels (gfx950): wave reductions, the FP64 MFMA 64 x 64 tile machinery
+// and the covariance functions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+
+#include "lk_kernels.h"
+
+namespace lk {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int KC = 32;          // K-chunk staged in LDS per step
+constexpr int LDSS = NB + 4;    // LDS row stride (doubles) of a KC x NB staging tile
+constexpr int TPB = 256;        // threads of the tile kernels (4 waves)
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ inline double readlane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// lower-triangular tile index t -> (i, j), i >= j, row-major over i
+__device__ inline void tri_decode(int t, int& i, int& j) {
+  int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((r + 1) * (r + 2) / 2 <= t) ++r;
+  while (r * (r + 1) / 2 > t) --r;
+  i = r;
+  j = t - r * (r + 1) / 2;
+}
+
+__device__ inline bool slot_live(const SlotParams& P, const SlotOut& O) {
+  return P.active && O.status != ST_CHOL_FAILED && O.status != ST_INACTIVE;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA tile machinery: a 64 x 64 output tile per 256-thread workgroup, each
+// wave owning a 32 x 32 quarter as 2 x 2 blocks of V_MFMA_F64_16X16X4_F64.
+// Operands are staged in LDS as [k][row] (row contiguous).  The instruction
+// computes D = P Q with P (16 x 4) given as lane l -> P(l & 15, l >> 4),
+// Q (4 x 16) as lane l -> Q(l >> 4, l & 15), and returns D(l >> 4 + 4e, l & 15)
+// in register e of lane l (checked on gfx950).  We feed P = B^T and Q = A^T,
+// so D = (A B)^T and lane l, register e holds C(row = l & 15, col = l >> 4 + 4e):
+// sixteen consecutive rows of one column per lane group, i.e. coalesced
+// column-major loads and stores of the C tile.
+// ---------------------------------------------------------------------------
+
+// Operand chunks are KC x NB slices of op(M) staged as dst[kk * LDSS + ii] =
+// op(M)(i0 + ii, k0 + kk):
+//   TRANS = false: op(M)(i, k) = M[k * ld + i]   (i contiguous in memory)
+//   TRANS = true : op(M)(i, k) = M[i * ld + k]   (k contiguous in memory)
+template <bool NEG>
+__device__ inline void mma_chunk(const double* __restrict__ sA, const double* __restrict__ sB, d4 (&acc)[2][2],
+                                 int lane, int wr, int wc) {
+  const int r = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int k0 = 0; k0 < KC; k0 += 4) {
+    const double* pa = sA + (k0 + kq) * LDSS + wr * 32 + r;
+    const double* pb = sB + (k0 + kq) * LDSS + wc * 32 + r;
+    double a0 = pa[0], a1 = pa[16];
+    const double b0 = pb[0], b1 = pb[16];
+    if (NEG) {
+      a0 = -a0;
+      a1 = -a1;
+    }
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// Register-staged copy of one KC x NB chunk: the global loads of the next
+// chunk are issued before the MFMAs of the current one and written to the
+// other LDS buffer afterwards (software pipelining over K).
+struct ChunkRegs {
+  double2 v[4];
+};
+
+template <bool TRANS>
+__device__ inline void chunk_load(ChunkRegs& r, const double* __restrict__ M, int ld, int i0, int k0, int tid) {
+  if (!TRANS) {
+    const int ii = 2 * (tid & 31);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      r.v[q] = *reinterpret_cast<const double2*>(M + (size_t)(k0 + (tid >> 5) + 8 * q) * ld + i0 + ii);
+  } else {
+    const int kk = 2 * (tid & 15);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      r.v[q] = *reinterpret_cast<const double2*>(M + (size_t)(i0 + (tid >> 4) + 16 * q) * ld + k0 + kk);
+  }
+}
+
+template <bool TRANS>
+__device__ inline void chunk_store(const ChunkRegs& r, double* __restrict__ dst, int tid) {
+  if (!TRANS) {
+    const int ii = 2 * (tid & 31);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kk = (tid >> 5) + 8 * q;
+      dst[kk * LDSS + ii] = r.v[q].x;
+      dst[kk * LDSS + ii + 1] = r.v[q].y;
+    }
+  } else {
+    const int kk = 2 * (tid & 15);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ii = (tid >> 4) + 16 * q;
+      dst[kk * LDSS + ii] = r.v[q].x;
+      dst[(kk + 1) * LDSS + ii] = r.v[q].y;
+    }
+  }
+}
+
+// acc += op(A)(ai0 + 0..63, k) op(B)(bj0 + 0..63, k)^T summed over
+// k in [kbeg, kend) (a multiple of KC), with op as above.  sbuf holds two
+// (A, B) chunk buffers: 4 * KC * LDSS doubles.  One barrier per chunk.
+template <bool TA, bool TB, bool NEG>
+__device__ inline void tile_mainloop(const double* __restrict__ A, int lda, int ai0, const double* __restrict__ B,
+                                     int ldb, int bj0, int kbeg, int kend, double* __restrict__ sbuf,
+                                     d4 (&acc)[2][2], int tid, int lane, int wr, int wc);
+
+__device__ inline void acc_zero(d4 (&acc)[2][2]) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+// row / col inside the 64 x 64 tile of accumulator element (bi, bj, e)
+__device__ inline int acc_row(int lane, int wr, int bi) { return wr * 32 + bi * 16 + (lane & 15); }
+__device__ inline int acc_col(int lane, int wc, int bj, int e) { return wc * 32 + bj * 16 + (lane >> 4) + 4 * e; }
+
+// C tile (column-major, leading dim ld) <-> accumulators
+__device__ inline void acc_load(d4 (&acc)[2][2], const double* __restrict__ C, int ld, int lane, int wr, int wc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        acc[bi][bj][e] = C[(size_t)acc_col(lane, wc, bj, e) * ld + acc_row(lane, wr, bi)];
+}
+
+__device__ inline void acc_store(const d4 (&acc)[2][2], double* __restrict__ C, int ld, int lane, int wr, int wc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        C[(size_t)acc_col(lane, wc, bj, e) * ld + acc_row(lane, wr, bi)] = acc[bi][bj][e];
+}
+
+template <bool TA, bool TB, bool NEG>
+__device__ inline void tile_mainloop(const double* __restrict__ A, int lda, int ai0, const double* __restrict__ B,
+                                     int ldb, int bj0, int kbeg, int kend, double* __restrict__ sbuf,
+                                     d4 (&acc)[2][2], int tid, int lane, int wr, int wc) {
+  if (kbeg >= kend) return;
+  double* sA[2] = {sbuf, sbuf + 2 * KC * LDSS};
+  double* sB[2] = {sbuf + KC * LDSS, sbuf + 3 * KC * LDSS};
+  ChunkRegs ra, rb;
+  chunk_load<TA>(ra, A, lda, ai0, kbeg, tid);
+  chunk_load<TB>(rb, B, ldb, bj0, kbeg, tid);
+  int buf = 0;
+  for (int k0 = kbeg; k0 < kend; k0 += KC) {
+    chunk_store<TA>(ra, sA[buf], tid);
+    chunk_store<TB>(rb, sB[buf], tid);
+    __syncthreads();
+    if (k0 + KC < kend) {  // next chunk's loads overlap this chunk's MFMAs
+      chunk_load<TA>(ra, A, lda, ai0, k0 + KC, tid);
+      chunk_load<TB>(rb, B, ldb, bj0, k0 + KC, tid);
+    }
+    mma_chunk<NEG>(sA[buf], sB[buf], acc, lane, wr, wc);
+    buf ^= 1;
+  }
+  // the caller may reuse sbuf: wait until every wave has finished reading it
+  __syncthreads();
+}
+
+// store the transpose: element (row, col) of the tile goes to C[row * ld + col]
+__device__ inline void acc_store_t(const d4 (&acc)[2][2], double* __restrict__ C, int ld, int lane, int wr,
+                                   int wc) {
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        C[(size_t)acc_row(lane, wr, bi) * ld + acc_col(lane, wc, bj, e)] = acc[bi][bj][e];
+}
+
+// ---------------------------------------------------------------------------
+// kernel functions k(dx; theta) and d ln k / d theta_k, restated from
+// src/lib/Covariance.cpp:24-179 (gauss, exp, matern3_2, matern5_2).  The
+// Matern forms use the exact identity exp(-sum(u - log1p(q(u)))) =
+// exp(-sum u) * prod q'(u) with q' = 1 + q, one exp per pair instead of one
+// exp and d log1p.
+// ---------------------------------------------------------------------------
+constexpr double SQRT3 = 1.7320508075688772;
+constexpr double SQRT5 = 2.2360679774997896;
+
+// rho(dx) for the kernel; it[k] = 1 / theta_k (entries k >= d hold dx = 0)
+template <int DT>
+__device__ inline double kern_rho(int kern, const double* dx, const double* it) {
+  double s = 0.0;
+  if (kern == K_GAUSS) {
+#pragma unroll
+    for (int k = 0; k < DT; ++k) {
+      const double v = dx[k] * it[k];
+      s = fma(v, v, s);
+    }
+    return exp(-0.5 * s);
+  } else if (kern == K_EXP) {
+#pragma unroll
+    for (int k = 0; k < DT; ++k) s += fabs(dx[k] * it[k]);
+    return exp(-s);
+  } else if (kern == K_MATERN32) {
+    double pr = 1.0;
+#pragma unroll
+    for (int k = 0; k < DT; ++k) {
+      const double u = SQRT3 * fabs(dx[k] * it[k]);
+      s += u;
+      pr *= 1.0 + u;
+    }
+    return exp(-s) * pr;
+  } else {
+    double pr = 1.0;
+#pragma unroll
+    for (int k = 0; k < DT; ++k) {
+      const double u = SQRT5 * fabs(dx[k] * it[k]);
+      s += u;
+      pr *= 1.0 + u + (u * u) * (1.0 / 3.0);
+    }
+    return exp(-s) * pr;
+  }
+}
+
+}  // namespace lk

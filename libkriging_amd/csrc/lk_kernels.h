@@ -127,4 +127,43 @@ struct PredictArgs {
 };
 hipError_t launch_predict(const PredictArgs& a, hipStream_t stream);
 
+// ---------------------------------------------------------------------------
+// NestedKriging "NK" aggregation (NestedKriging::predict_nk,
+// src/lib/NestedKriging.cpp:353-465).  The design is stored group by group,
+// each group's rows padded to a multiple of 64 (pad rows: X = 0, weights 0),
+// so that every 64-row tile belongs to exactly one group.
+// ---------------------------------------------------------------------------
+struct NkGroup {
+  int off;      // first row of the group in the group-ordered arrays
+  int n;        // points in the group
+  int npad;     // n rounded up to 64
+  int pad_;
+  long long loff;  // offset of the group's L^-1 (npad x npad) in the pool
+};
+
+struct NkArgs {
+  const double* Xg;        // Npad x d, group-ordered design, ld = Npad
+  const double* Linv;      // pool of per-group L^-1 (lower, column-major)
+  double* alpha;           // Npad: R_g^-1 (y_g - beta0), group-ordered
+  const NkGroup* groups;   // p
+  const int* tile_group;   // Npad / 64: group of each 64-row tile
+  const int2* pairs;       // npairs (i < j) group pairs
+  const double* Xc;        // qc x d points of this chunk, ld = qc
+  double* C;               // Npad x qpad: k(x_a, x_t), then R_g^-1 k_g
+  double* U;               // Npad x qpad: L_g^-1 k_g
+  double* M;               // qc x p: submodel means
+  double* Kd;              // qc x p: k_g' R_g^-1 k_g
+  double* KM;              // qc x p x p: cov(M_i, M_j) / sigma2 (column-major per point)
+  double* KF;              // qc x p x p: factor workspace
+  double* out;             // 2 qc: mean, variance
+  int* info;               // qc: 0 Cholesky, 1 LU fallback, 2 singular
+  double theta[DMAX];
+  double beta0, sigma2, jitter;
+  int Npad, d, p, qc, qpad, kernel, npairs;
+};
+hipError_t launch_nk_prepare(const NkArgs& a, hipStream_t stream);  // C, U, M, Kd, R_g^-1 k_g
+hipError_t launch_nk_pairs(const NkArgs& a, hipStream_t stream);    // off-diagonal KM
+hipError_t launch_nk_solve(const NkArgs& a, hipStream_t stream);    // per-point aggregation
+hipError_t launch_nk_alpha(const NkArgs& a, const double* r, hipStream_t stream);  // alpha = Linv' r
+
 }  // namespace lk

@@ -24,199 +24,10 @@
 
 #include <math.h>
 
+#include "lk_device.hpp"
 #include "lk_kernels.h"
 
 namespace lk {
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-constexpr int KC = 32;          // K-chunk staged in LDS per step
-constexpr int LDSS = NB + 4;    // LDS row stride (doubles) of a KC x NB staging tile
-constexpr int TPB = 256;        // threads of the tile kernels (4 waves)
-
-// ---------------------------------------------------------------------------
-// small helpers
-// ---------------------------------------------------------------------------
-__device__ inline double readlane_d(double v, int lane) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-  return __hiloint2double(hi, lo);
-}
-
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-__device__ inline double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-// lower-triangular tile index t -> (i, j), i >= j, row-major over i
-__device__ inline void tri_decode(int t, int& i, int& j) {
-  int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-  while ((r + 1) * (r + 2) / 2 <= t) ++r;
-  while (r * (r + 1) / 2 > t) --r;
-  i = r;
-  j = t - r * (r + 1) / 2;
-}
-
-__device__ inline bool slot_live(const SlotParams& P, const SlotOut& O) {
-  return P.active && O.status != ST_CHOL_FAILED && O.status != ST_INACTIVE;
-}
-
-// ---------------------------------------------------------------------------
-// MFMA tile machinery: a 64 x 64 output tile per 256-thread workgroup, each
-// wave owning a 32 x 32 quarter as 2 x 2 blocks of V_MFMA_F64_16X16X4_F64.
-// Operands are staged in LDS as [k][row] (row contiguous).  The instruction
-// computes D = P Q with P (16 x 4) given as lane l -> P(l & 15, l >> 4),
-// Q (4 x 16) as lane l -> Q(l >> 4, l & 15), and returns D(l >> 4 + 4e, l & 15)
-// in register e of lane l (checked on gfx950).  We feed P = B^T and Q = A^T,
-// so D = (A B)^T and lane l, register e holds C(row = l & 15, col = l >> 4 + 4e):
-// sixteen consecutive rows of one column per lane group, i.e. coalesced
-// column-major loads and stores of the C tile.
-// ---------------------------------------------------------------------------
-
-// Stage a KC x NB chunk of op(M) into dst[kk * LDSS + ii] = op(M)(i0 + ii, k0 + kk).
-//   TRANS = false: op(M)(i, k) = M[k * ld + i]   (i contiguous in memory)
-//   TRANS = true : op(M)(i, k) = M[i * ld + k]   (k contiguous in memory)
-template <bool TRANS>
-__device__ inline void stage(double* __restrict__ dst, const double* __restrict__ M, int ld, int i0, int k0,
-                             int tid) {
-  if (!TRANS) {
-    const int ii = 2 * (tid & 31);
-#pragma unroll
-    for (int r = 0; r < KC / 8; ++r) {
-      const int kk = (tid >> 5) + 8 * r;
-      const double2 v = *reinterpret_cast<const double2*>(M + (size_t)(k0 + kk) * ld + i0 + ii);
-      dst[kk * LDSS + ii] = v.x;
-      dst[kk * LDSS + ii + 1] = v.y;
-    }
-  } else {
-    const int kk = 2 * (tid & 15);
-#pragma unroll
-    for (int r = 0; r < NB / 16; ++r) {
-      const int ii = (tid >> 4) + 16 * r;
-      const double2 v = *reinterpret_cast<const double2*>(M + (size_t)(i0 + ii) * ld + k0 + kk);
-      dst[kk * LDSS + ii] = v.x;
-      dst[(kk + 1) * LDSS + ii] = v.y;
-    }
-  }
-}
-
-template <bool NEG>
-__device__ inline void mma_chunk(const double* __restrict__ sA, const double* __restrict__ sB, d4 (&acc)[2][2],
-                                 int lane, int wr, int wc) {
-  const int r = lane & 15, kq = lane >> 4;
-#pragma unroll
-  for (int k0 = 0; k0 < KC; k0 += 4) {
-    const double* pa = sA + (k0 + kq) * LDSS + wr * 32 + r;
-    const double* pb = sB + (k0 + kq) * LDSS + wc * 32 + r;
-    double a0 = pa[0], a1 = pa[16];
-    const double b0 = pb[0], b1 = pb[16];
-    if (NEG) {
-      a0 = -a0;
-      a1 = -a1;
-    }
-    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
-    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[0][1], 0, 0, 0);
-    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[1][0], 0, 0, 0);
-    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
-  }
-}
-
-__device__ inline void acc_zero(d4 (&acc)[2][2]) {
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-}
-
-// row / col inside the 64 x 64 tile of accumulator element (bi, bj, e)
-__device__ inline int acc_row(int lane, int wr, int bi) { return wr * 32 + bi * 16 + (lane & 15); }
-__device__ inline int acc_col(int lane, int wc, int bj, int e) { return wc * 32 + bj * 16 + (lane >> 4) + 4 * e; }
-
-// C tile (column-major, leading dim ld) <-> accumulators
-__device__ inline void acc_load(d4 (&acc)[2][2], const double* __restrict__ C, int ld, int lane, int wr, int wc) {
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        acc[bi][bj][e] = C[(size_t)acc_col(lane, wc, bj, e) * ld + acc_row(lane, wr, bi)];
-}
-
-__device__ inline void acc_store(const d4 (&acc)[2][2], double* __restrict__ C, int ld, int lane, int wr, int wc) {
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        C[(size_t)acc_col(lane, wc, bj, e) * ld + acc_row(lane, wr, bi)] = acc[bi][bj][e];
-}
-
-// store the transpose: element (row, col) of the tile goes to C[row * ld + col]
-__device__ inline void acc_store_t(const d4 (&acc)[2][2], double* __restrict__ C, int ld, int lane, int wr,
-                                   int wc) {
-#pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        C[(size_t)acc_row(lane, wr, bi) * ld + acc_col(lane, wc, bj, e)] = acc[bi][bj][e];
-}
-
-// ---------------------------------------------------------------------------
-// kernel functions k(dx; theta) and d ln k / d theta_k, restated from
-// src/lib/Covariance.cpp:24-179 (gauss, exp, matern3_2, matern5_2).  The
-// Matern forms use the exact identity exp(-sum(u - log1p(q(u)))) =
-// exp(-sum u) * prod q'(u) with q' = 1 + q, one exp per pair instead of one
-// exp and d log1p.
-// ---------------------------------------------------------------------------
-constexpr double SQRT3 = 1.7320508075688772;
-constexpr double SQRT5 = 2.2360679774997896;
-
-// rho(dx) for the kernel; it[k] = 1 / theta_k (entries k >= d hold dx = 0)
-template <int DT>
-__device__ inline double kern_rho(int kern, const double* dx, const double* it) {
-  double s = 0.0;
-  if (kern == K_GAUSS) {
-#pragma unroll
-    for (int k = 0; k < DT; ++k) {
-      const double v = dx[k] * it[k];
-      s = fma(v, v, s);
-    }
-    return exp(-0.5 * s);
-  } else if (kern == K_EXP) {
-#pragma unroll
-    for (int k = 0; k < DT; ++k) s += fabs(dx[k] * it[k]);
-    return exp(-s);
-  } else if (kern == K_MATERN32) {
-    double pr = 1.0;
-#pragma unroll
-    for (int k = 0; k < DT; ++k) {
-      const double u = SQRT3 * fabs(dx[k] * it[k]);
-      s += u;
-      pr *= 1.0 + u;
-    }
-    return exp(-s) * pr;
-  } else {
-    double pr = 1.0;
-#pragma unroll
-    for (int k = 0; k < DT; ++k) {
-      const double u = SQRT5 * fabs(dx[k] * it[k]);
-      s += u;
-      pr *= 1.0 + u + (u * u) * (1.0 / 3.0);
-    }
-    return exp(-s) * pr;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // (1) covariance build: lower 64 x 64 tiles (diagonal tiles in full)
@@ -517,17 +328,11 @@ __global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ sl
   if (!slot_live(params[b], out[b]) || r >= S.nblk) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const int i0 = r * NB, k0 = k * NB, ld = S.npad;
-  __shared__ double sA[KC * LDSS];
-  __shared__ double sB[KC * LDSS];
+  __shared__ double sbuf[4 * KC * LDSS];
   d4 acc[2][2];
   acc_zero(acc);
-  for (int kc = 0; kc < NB; kc += KC) {
-    stage<false>(sA, S.A, ld, i0, k0 + kc, tid);  // A(i0 + ii, k0 + kc + kk)
-    stage<false>(sB, S.W, ld, k0, k0 + kc, tid);  // Dinv(c, kc + kk) at W[(k0+kc+kk)*ld + k0 + c]
-    __syncthreads();
-    mma_chunk<false>(sA, sB, acc, lane, wr, wc);
-    __syncthreads();
-  }
+  // A(i0 + ii, k) times Dinv(c, k) at W[k * ld + k0 + c]
+  tile_mainloop<false, false, false>(S.A, ld, i0, S.W, ld, k0, k0, k0 + NB, sbuf, acc, tid, lane, wr, wc);
   acc_store(acc, S.A + (size_t)k0 * ld + i0, ld, lane, wr, wc);
 }
 
@@ -547,18 +352,11 @@ __global__ __launch_bounds__(TPB) void k_potrf_update(const Slot* __restrict__ s
   tri_decode(blockIdx.x, ti, tj);
   const int i0 = (k + 1 + ti) * NB, j0 = (k + 1 + tj) * NB, k0 = k * NB, ld = S.npad;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
-  __shared__ double sA[KC * LDSS];
-  __shared__ double sB[KC * LDSS];
+  __shared__ double sbuf[4 * KC * LDSS];
   d4 acc[2][2];
   double* C = S.A + (size_t)j0 * ld + i0;
   acc_load(acc, C, ld, lane, wr, wc);
-  for (int kc = 0; kc < NB; kc += KC) {
-    stage<false>(sA, S.A, ld, i0, k0 + kc, tid);
-    stage<false>(sB, S.A, ld, j0, k0 + kc, tid);
-    __syncthreads();
-    mma_chunk<true>(sA, sB, acc, lane, wr, wc);
-    __syncthreads();
-  }
+  tile_mainloop<false, false, true>(S.A, ld, i0, S.A, ld, j0, k0, k0 + NB, sbuf, acc, tid, lane, wr, wc);
   acc_store(acc, C, ld, lane, wr, wc);
 }
 
@@ -584,32 +382,19 @@ __global__ __launch_bounds__(TPB) void k_trtri(const Slot* __restrict__ slots, c
   const int ld = S.npad;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const int i0 = it * NB, j0 = jt * NB;
-  __shared__ double sA[KC * LDSS];
-  __shared__ double sB[KC * LDSS];
+  __shared__ double sbuf[4 * KC * LDSS];
   d4 acc[2][2];
   acc_zero(acc);
   if (step == 0) {
-    // T(i, j) = sum_{kt = jt}^{top + s - 1} L(i, kt) Linv(kt, j)
-    for (int kt = jt; kt < top + s; ++kt)
-      for (int kc = 0; kc < NB; kc += KC) {
-        stage<false>(sA, S.A, ld, i0, kt * NB + kc, tid);
-        stage<true>(sB, S.W, ld, j0, kt * NB + kc, tid);
-        __syncthreads();
-        mma_chunk<false>(sA, sB, acc, lane, wr, wc);
-        __syncthreads();
-      }
+    // T(i, j) = sum_{k = jt*NB}^{(top + s)*NB - 1} L(i, k) Linv(k, j)
+    tile_mainloop<false, true, false>(S.A, ld, i0, S.W, ld, j0, jt * NB, (top + s) * NB, sbuf, acc, tid, lane, wr,
+                                      wc);
     // T(i, j) stored at W[(i0 + row) * ld + j0 + col] (strict upper triangle of W)
     acc_store_t(acc, S.W + (size_t)i0 * ld + j0, ld, lane, wr, wc);
   } else {
-    // Linv(i, j) = -sum_{kt = bot}^{it} Linv(i, kt) T(kt, j)
-    for (int kt = bot; kt <= it; ++kt)
-      for (int kc = 0; kc < NB; kc += KC) {
-        stage<false>(sA, S.W, ld, i0, kt * NB + kc, tid);
-        stage<false>(sB, S.W, ld, j0, kt * NB + kc, tid);
-        __syncthreads();
-        mma_chunk<true>(sA, sB, acc, lane, wr, wc);
-        __syncthreads();
-      }
+    // Linv(i, j) = -sum_{k = bot*NB}^{(it + 1)*NB - 1} Linv(i, k) T(k, j)
+    tile_mainloop<false, false, true>(S.W, ld, i0, S.W, ld, j0, bot * NB, (it + 1) * NB, sbuf, acc, tid, lane, wr,
+                                      wc);
     acc_store(acc, S.W + (size_t)j0 * ld + i0, ld, lane, wr, wc);
   }
 }
@@ -630,6 +415,7 @@ __device__ inline void lower_matvec_block(const Slot& S, int rb, const double* _
   for (int c0 = 0; c0 < ncols; c0 += 4) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const int nc = min(4, ncols - c0);
+#pragma unroll 8
     for (int k = w; k < kend; k += 4) {
       const double lv = S.W[(size_t)k * ld + i0 + lane];
 #pragma unroll
@@ -771,6 +557,7 @@ __global__ __launch_bounds__(TPB) void k_solve_x(const Slot* __restrict__ slots,
   for (int cc = w; cc < NB; cc += 4) {
     const int c = blockIdx.x * NB + cc;
     double s = 0.0, aw = 0.0, al = 0.0;
+#pragma unroll 4
     for (int i = c + lane; i < ld; i += 64) {
       const double wv = S.W[(size_t)c * ld + i];
       s = fma(wv, e[i], s);
@@ -882,22 +669,15 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
   const int ld = S.npad, n = S.n, d = S.d;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const int i0 = it * NB, j0 = jt * NB;
-  __shared__ double sA[KC * LDSS];
-  __shared__ double sB[KC * LDSS];
+  __shared__ double sbuf[4 * KC * LDSS];
   d4 acc[2][2];
   acc_zero(acc);
-  for (int kt = it; kt < S.nblk; ++kt)
-    for (int kc = 0; kc < NB; kc += KC) {
-      stage<true>(sA, S.W, ld, i0, kt * NB + kc, tid);  // Linv(kt*NB + kc + kk, i0 + ii)
-      stage<true>(sB, S.W, ld, j0, kt * NB + kc, tid);  // Linv(kt*NB + kc + kk, j0 + jj)
-      __syncthreads();
-      mma_chunk<false>(sA, sB, acc, lane, wr, wc);
-      __syncthreads();
-    }
+  // Linv(k, i0 + ii) Linv(k, j0 + jj) over k >= it * NB
+  tile_mainloop<true, true, false>(S.W, ld, i0, S.W, ld, j0, it * NB, S.nblk * NB, sbuf, acc, tid, lane, wr, wc);
 
-  // epilogue: X rows of both blocks and the x vector in LDS (reuse sA / sB)
-  double* sXi = sA;  // d x NB
-  double* sXj = sB;
+  // epilogue: X rows of both blocks and the x vector in LDS (reuse sbuf)
+  double* sXi = sbuf;  // d x NB
+  double* sXj = sbuf + DMAX * NB;
   __shared__ double sxi[NB], sxj[NB];
   const double* xv = S.V + (size_t)(S.p + 3) * ld;
   for (int q = tid; q < d * NB; q += TPB) {
@@ -1034,24 +814,29 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
 // ---------------------------------------------------------------------------
 // (7) gradient (Kriging.cpp:303-326): sums of the tile partials in tile order
 // ---------------------------------------------------------------------------
-__global__ void k_finalize_grad(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
-                                SlotOut* __restrict__ out, int nslots) {
+__global__ __launch_bounds__(TPB) void k_finalize_grad(const Slot* __restrict__ slots,
+                                                       const SlotParams* __restrict__ params,
+                                                       SlotOut* __restrict__ out, int nslots) {
   const int b = blockIdx.x;
   if (b >= nslots) return;
   const Slot S = slots[b];
   const SlotParams& P = params[b];
   SlotOut& O = out[b];
   if (!slot_live(P, O) || !P.want_grad) return;
-  const int q = threadIdx.x;  // 0 .. GRAD_STRIDE-1
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int ntiles = S.nblk * (S.nblk + 1) / 2;
   __shared__ double tot[GRAD_STRIDE];
-  if (q < GRAD_STRIDE) {
-    double s = 0.0;
+  // wave w sums components w, w + 4, ...; lanes stride over tiles, then a
+  // fixed-order butterfly: deterministic for a given tile count
+  for (int q = w; q < GRAD_STRIDE; q += TPB / 64) {
     const double* src = S.red + red_grad_off(S.nblk) + q;
-    for (int t = 0; t < ntiles; ++t) s += src[(size_t)t * GRAD_STRIDE];
-    tot[q] = s;
+    double s = 0.0;
+    for (int t = lane; t < ntiles; t += 64) s += src[(size_t)t * GRAD_STRIDE];
+    s = wave_sum(s);
+    if (lane == 0) tot[q] = s;
   }
   __syncthreads();
+  const int q = threadIdx.x;
   if (q != 0) return;
   const int d = S.d;
   const double s2g = O.sigma2;
@@ -1271,18 +1056,11 @@ __global__ __launch_bounds__(TPB) void k_trmm_lower(PredictArgs a) {
   const int jt = blockIdx.x, it = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const int i0 = it * NB, j0 = jt * NB, ld = a.npad;
-  __shared__ double sA[KC * LDSS];
-  __shared__ double sB[KC * LDSS];
+  __shared__ double sbuf[4 * KC * LDSS];
   d4 acc[2][2];
   acc_zero(acc);
-  for (int kt = 0; kt <= it; ++kt)
-    for (int kc = 0; kc < NB; kc += KC) {
-      stage<false>(sA, a.Linv, ld, i0, kt * NB + kc, tid);  // Linv(i0 + ii, k)
-      stage<true>(sB, a.Ron, ld, j0, kt * NB + kc, tid);    // R_on(k, j0 + jj)
-      __syncthreads();
-      mma_chunk<false>(sA, sB, acc, lane, wr, wc);
-      __syncthreads();
-    }
+  // Linv(i0 + ii, k) R_on(k, j0 + jj) over k < (it + 1) * NB
+  tile_mainloop<false, true, false>(a.Linv, ld, i0, a.Ron, ld, j0, 0, (it + 1) * NB, sbuf, acc, tid, lane, wr, wc);
   acc_store(acc, a.C + (size_t)j0 * ld + i0, ld, lane, wr, wc);
 }
 
@@ -1427,8 +1205,7 @@ hipError_t launch_lauum_adjoint(const Batch& B) {
   return hipGetLastError();
 }
 hipError_t launch_finalize_grad(const Batch& B) {
-  hipLaunchKernelGGL(k_finalize_grad, dim3(B.nslots), dim3(GRAD_STRIDE), 0, B.stream, B.slots, B.params, B.out,
-                     B.nslots);
+  hipLaunchKernelGGL(k_finalize_grad, dim3(B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, B.nslots);
   return hipGetLastError();
 }
 hipError_t launch_rcond_estimate(const Batch& B) {

@@ -350,12 +350,12 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
     Q.nugget = R.flags.nugget;
     Q.num_nugget = chol.num_nugget;
     Q.min_rcond = chol.min_rcond;
-    Q.inc = 0;
+    Q.inc = R.flags.inc0;
     Q.active = R.active ? 1 : 0;
     if (!R.active) res[s].ll = std::numeric_limits<double>::quiet_NaN();
     Q.est_sigma2 = R.flags.est_sigma2;
     Q.est_nugget = R.flags.est_nugget;
-    Q.rcond_check = chol.rcond_check;
+    Q.rcond_check = R.flags.rcond_check >= 0 ? R.flags.rcond_check : chol.rcond_check;
     Q.want_grad = want_grad;
     Q.gamma_has_alpha = R.gamma_n > P.d;
   }
@@ -387,7 +387,8 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
       }
       // failed factorization or rcond below threshold: next step of the
       // ladder (LinearAlgebra.cpp:78-90)
-      if (Q.inc > chol.max_inc) {
+      const int max_inc = reqs[s].flags.max_inc >= 0 ? reqs[s].flags.max_inc : chol.max_inc;
+      if (Q.inc > max_inc) {
         E.status = EVAL_CHOL_FAILED;
         E.inc = Q.inc;
         Q.active = 0;

@@ -38,6 +38,11 @@ struct ModelFlags {
   double sigma2 = 1.0;
   double nugget = 0.0;
   double alpha = 1.0;  // member alpha used when gamma has d entries
+  // factorisation overrides of one request (NestedKriging's chol(R + 1e-10 I)
+  // is inc0 = 1, max_inc = 0, rcond_check = 0: one fixed jitter, no retry)
+  int inc0 = 0;          // jitter ladder step of the first attempt
+  int max_inc = -1;      // last step retried (-1: CholConfig::max_inc)
+  int rcond_check = -1;  // -1: CholConfig::rcond_check
 };
 
 struct EvalRequest {
