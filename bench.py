@@ -58,6 +58,30 @@ def cpu_baseline(X, y, theta, gpu_ll, gpu_grad, budget_s=15.0, max_evals=20):
                       f"single-threaded C oracle"}, ll_err, g_err
 
 
+def cholesky_large(N, n=8192, d=10, reps=3):
+    """The single-evaluation Cholesky at n = 8192 (gauss kernel, d = 10,
+    theta log-U[0.3, 1.5], nugget ratio 1e-6), timed live over its own
+    launches: n^3 / 3 flops per factorisation against the FP64 MFMA peak."""
+    rng = np.random.default_rng(125)
+    X = rng.uniform(size=(n, d))
+    y = rng.normal(size=n)
+    theta = np.exp(rng.uniform(np.log(0.3), np.log(1.5), size=d))
+    gam = np.concatenate([theta, [1.0 / (1.0 + 1e-6)]])[None, :]
+    ctx = N.Context(0)
+    pid = ctx.problem(X, y, np.ones((n, 1)), "gauss", "nugget")
+    ctx.ll_grad_batch([pid], gam, False)  # pool allocation and graph capture
+    ctx.set_timed_stage("cholesky")
+    for _ in range(reps):
+        _, _, st, _ = ctx.ll_grad_batch([pid], gam, False)
+    ms, count = ctx.take_stage_time()
+    ctx.set_timed_stage(-1)
+    assert (st == 0).all()
+    tflops = n ** 3 / 3.0 / (ms * 1e-3) / 1e12
+    ctx.drop_problem(pid)
+    return {"n": n, "avg_ms": ms, "runs": count, "achieved_tflops": tflops,
+            "frac_of_fp64_mfma_peak": tflops / MFMA_FP64_PEAK_TFLOPS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +89,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-chol", action="store_true", help="skip the n = 8192 Cholesky measurement")
     ap.add_argument("--stage", default="adjoint", help="kernel timed live for the roofline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help=f"also run a {COPY_BYTES >> 20} MiB device copy (counter calibration)")
@@ -150,6 +175,8 @@ def main():
         "plain_equals_batched": same,
         "roofline": roof,
     }
+    if not args.no_chol:
+        line["cholesky_n8192"] = cholesky_large(N)
     if not args.no_cpu:
         cb, ll_err, g_err = cpu_baseline(X, y, theta, ll, grad)
         line["cpu_baseline"] = cb

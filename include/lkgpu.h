@@ -213,6 +213,58 @@ int lk_lbfgsb_minimize(int n, double* x, const double* lower, const double* uppe
                        int max_iter, double pgtol, double factr, int m, double* f_out, int* iters_out,
                        int* evals_out, char* task_out, int task_len);
 
+/* ======================================================================
+ * Layer 3: NestedKriging, mirroring bindings/Julia/jlibkriging/csrc/
+ * libkriging_c.h:394-456 (implementation libkriging_c.cpp:1663-1870).
+ * Plain Kriging submodels only: n_warping must be 0 and the LLVecchia
+ * objective is rejected (-1 with a message).  All groups' submodel fits run
+ * in shared batched pipelines; the NK aggregation runs on the device.
+ * Partitions: "kmeans" / "random" seeded by `seed` through std::mt19937
+ * (reproducible per seed, not Armadillo's generator).
+ * ====================================================================== */
+void* lk_nested_kriging_new_fit(const double* y, int n, const double* X, int nX, int d, const char* kernel,
+                                int nb_groups, const char* aggregation, const char* partition, int seed,
+                                const char** warping, int n_warping, const char* regmodel, const char* optim,
+                                const char* objective, const double* sigma2, int is_sigma2_estim, const double* theta,
+                                int theta_n, int is_theta_estim, const double* beta, int beta_n, int is_beta_estim);
+void lk_nested_kriging_delete(void* ptr);
+int lk_nested_kriging_fit(void* ptr, const double* y, int n, const double* X, int nX, int d, int nb_groups,
+                          const char** warping, int n_warping, const char* regmodel, const char* optim,
+                          const char* objective, const double* sigma2, int is_sigma2_estim, const double* theta,
+                          int theta_n, int is_theta_estim, const double* beta, int beta_n, int is_beta_estim);
+int lk_nested_kriging_predict(void* ptr, const double* X_n, int m, int d, int return_stdev, double* mean_out,
+                              double* stdev_out);
+const char* lk_nested_kriging_summary(void* ptr);
+const char* lk_nested_kriging_kernel(void* ptr);
+const char* lk_nested_kriging_aggregation(void* ptr);
+int lk_nested_kriging_nb_groups(void* ptr);
+int lk_nested_kriging_get_theta(void* ptr, double* out, int* n);
+double lk_nested_kriging_get_sigma2(void* ptr);
+double lk_nested_kriging_get_beta0(void* ptr);
+
+/* Extensions with no counterpart in the reference's C ABI:
+ * lk_nested_kriging_new is the NestedKriging(covType) constructor
+ * (NestedKriging.hpp:62) plus the aggregation / partition / seed options;
+ * lk_nested_kriging_fit_groups fits on caller-given groups (group_of[i] in
+ * 0..nb_groups-1, every group non-empty), e.g. uneven group sizes;
+ * lk_nested_kriging_groups writes the group index of every point;
+ * lk_nested_kriging_submodel returns a borrowed lk_kriging_* handle of
+ * submodel i (valid until the next fit or delete; never delete it);
+ * lk_nested_kriging_set_predict_chunk is NestedKriging::set_predict_chunk
+ * (0 = sized from a device-memory budget); lk_nested_kriging_predict_fallbacks
+ * counts the points of the last NK predict whose aggregation system was not
+ * positive definite (solved by LU, or by the pseudo-inverse when singular). */
+void* lk_nested_kriging_new(const char* kernel, const char* aggregation, const char* partition, int seed);
+int lk_nested_kriging_fit_groups(void* ptr, const double* y, int n, const double* X, int nX, int d,
+                                 const int* group_of, int nb_groups, const char* regmodel, const char* optim,
+                                 const char* objective, const double* sigma2, int is_sigma2_estim,
+                                 const double* theta, int theta_n, int is_theta_estim, const double* beta,
+                                 int beta_n, int is_beta_estim);
+int lk_nested_kriging_groups(void* ptr, int* group_of_out);
+void* lk_nested_kriging_submodel(void* ptr, int i);
+int lk_nested_kriging_set_predict_chunk(void* ptr, int chunk);
+int lk_nested_kriging_predict_fallbacks(void* ptr);
+
 #ifdef __cplusplus
 }
 #endif

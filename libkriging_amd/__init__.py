@@ -3,13 +3,15 @@
 concentrated LL and its adjoint gradient inside the L-BFGS-B fit), written
 for one MI355X (gfx950) in FP64 HIP behind a C ABI (include/lkgpu.h).
 
-Python surface mirrors pylibkriging: ``Kriging`` (= lk.Kriging), ``Optim``;
+Python surface mirrors pylibkriging: ``Kriging`` (= lk.Kriging),
+``NestedKriging`` (= lk.NestedKriging, plain submodels), ``Optim``;
 plus ``Context`` (the batched low-level API) and, when torch is importable,
 ``KrigingLogLikelihood`` (a torch.autograd.Function whose backward is the
 adjoint gradient).
 """
 from ._native import Context, ModelFlags  # noqa: F401
 from .kriging import Kriging, Optim, lbfgsb_minimize, set_num_nugget, check_chol_rcond, reset_seed  # noqa: F401
+from .nested import NestedKriging  # noqa: F401
 
 __version__ = "0.1.0"
 

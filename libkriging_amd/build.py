@@ -16,12 +16,14 @@ LIB = os.path.join(HERE, "liblkgpu.so")
 
 SOURCES = [
     "lk_kernels.hip",
-    "lk_predict.hip",
+    "lk_nested.hip",
     "lk_runtime.cpp",
     "lk_capi.cpp",
     "lbfgsb.cpp",
     "kriging.cpp",
     "kriging_capi.cpp",
+    "nested.cpp",
+    "nested_capi.cpp",
 ]
 
 ARCH = os.environ.get("LKGPU_ARCH", "gfx950")

@@ -126,9 +126,18 @@ class Kriging:
         self.fit(y, X, vals.get("regmodel", "constant"), vals.get("normalize", False), vals.get("optim", "BFGS"),
                  vals.get("objective", "LL"), vals.get("parameters", None))
 
+    @classmethod
+    def _borrowed(cls, h, owner):
+        """a view of a model owned by another object (kept alive through owner)"""
+        k = cls.__new__(cls)
+        k._h = h
+        k._owned = False
+        k._keep = owner
+        return k
+
     def __del__(self):
         try:
-            if self._h:
+            if self._h and getattr(self, "_owned", True):
                 N.lib().lk_kriging_delete(self._h)
                 self._h = None
         except Exception:

@@ -134,6 +134,22 @@ void lko_build_R(int kernel, const double* X, int n, int d, const double* theta,
   if (dxp != dx) free(dxp);
 }
 
+/* Cross-correlation matrix out[i + j*n1] = k(x1_i - x2_j; theta), as
+ * NestedKriging::corrMat (src/lib/NestedKriging.cpp:105-118): the kernel on
+ * the difference vector, no coincident-point special case.  X1 is n1 x d and
+ * X2 n2 x d, both column-major. */
+void lko_corr_mat(int kernel, const double* X1, int n1, const double* X2, int n2, int d, const double* theta,
+                  double* out) {
+  double dx[64];
+  double* dxp = d <= 64 ? dx : (double*)malloc(sizeof(double) * d);
+  for (int j = 0; j < n2; j++)
+    for (int i = 0; i < n1; i++) {
+      for (int k = 0; k < d; k++) dxp[k] = X1[i + (size_t)k * n1] - X2[j + (size_t)k * n2];
+      out[i + (size_t)j * n1] = lko_cov(kernel, dxp, theta, d);
+    }
+  if (dxp != dx) free(dxp);
+}
+
 /* Lower Cholesky in place (LAPACK dpotrf semantics: returns j+1 when the
  * j-th pivot is <= 0 or NaN, 0 on success; the strict upper triangle is set to
  * zero as arma::chol(L, X, "lower") returns a triangular matrix). */

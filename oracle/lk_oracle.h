@@ -67,6 +67,9 @@ void lko_dlncov_dtheta(int kernel, const double* dx, const double* theta, int d,
 
 void lko_build_R(int kernel, const double* X, int n, int d, const double* theta, double factor,
                  const double* diag, double* R);
+/* NestedKriging::corrMat: out (n1 x n2) = k(x1_i - x2_j) */
+void lko_corr_mat(int kernel, const double* X1, int n1, const double* X2, int n2, int d, const double* theta,
+                  double* out);
 
 int lko_potrf_lower(double* A, int n);
 double lko_trcon_lower(const double* L, int n);

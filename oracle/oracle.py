@@ -46,6 +46,8 @@ def lib():
         L.lko_cov.restype = ctypes.c_double
         L.lko_cov.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int]
         L.lko_dlncov_dtheta.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp]
+        L.lko_corr_mat.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.lko_corr_mat.restype = None
         L.lko_build_R.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int, _dp,
                                   ctypes.c_double, _dp, _dp]
         L.lko_potrf_lower.restype = ctypes.c_int
@@ -151,6 +153,17 @@ def cov(kernel, dx, theta):
     dx = _f64(dx)
     theta = _f64(theta)
     return lib().lko_cov(KERNELS[kernel], _ptr(dx), _ptr(theta), dx.size)
+
+
+def corr_mat(kernel, X1, X2, theta):
+    """NestedKriging::corrMat: k(x1_i - x2_j), n1 x n2"""
+    X1 = _f64(np.asarray(X1, dtype=np.float64))
+    X2 = _f64(np.asarray(X2, dtype=np.float64))
+    theta = _f64(theta)
+    out = np.zeros((X1.shape[0], X2.shape[0]), order="F")
+    lib().lko_corr_mat(KERNELS[kernel], _ptr(X1), X1.shape[0], _ptr(X2), X2.shape[0], X1.shape[1], _ptr(theta),
+                       _ptr(out))
+    return out
 
 
 def dlncov_dtheta(kernel, dx, theta):
