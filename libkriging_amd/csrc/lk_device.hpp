@@ -46,6 +46,22 @@ __device__ inline void tri_decode(int t, int& i, int& j) {
   j = t - r * (r + 1) / 2;
 }
 
+// (tile, slot) of this workgroup in a (tiles, slots) grid, XCD-aware.
+// Workgroups go to the 8 XCDs round-robin by linear id, each XCD with its own
+// L2: with slots a multiple of 8, slot s is placed on XCD s % 8 and its tiles
+// in consecutive dispatch order there, so one slot's matrices are shared in
+// one L2 instead of being spread over all eight.  Bijective on the grid.
+struct TileSlot {
+  int tile, slot;
+};
+__device__ inline TileSlot tile_slot() {
+  const int T = gridDim.x, S = gridDim.y;
+  if ((S & 7) != 0) return TileSlot{(int)blockIdx.x, (int)blockIdx.y};
+  const int L = blockIdx.x + T * blockIdx.y;
+  const int x = L & 7, idx = L >> 3;
+  return TileSlot{idx % T, x + 8 * (idx / T)};
+}
+
 __device__ inline bool slot_live(const SlotParams& P, const SlotOut& O) {
   return P.active && O.status != ST_CHOL_FAILED && O.status != ST_INACTIVE;
 }

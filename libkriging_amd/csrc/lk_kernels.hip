@@ -36,18 +36,19 @@ namespace lk {
 template <int DT>
 __global__ __launch_bounds__(TPB) void k_build(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
                                                SlotOut* __restrict__ out) {
-  const int b = blockIdx.y;
+  const TileSlot ts = tile_slot();
+  const int b = ts.slot;
   const Slot S = slots[b];
   const SlotParams& P = params[b];
   if (!P.active) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[b].status = ST_INACTIVE;
+    if (ts.tile == 0 && threadIdx.x == 0) out[b].status = ST_INACTIVE;
     return;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[b].status = ST_OK;
+  if (ts.tile == 0 && threadIdx.x == 0) out[b].status = ST_OK;
   const int ntiles = S.nblk * (S.nblk + 1) / 2;
-  if ((int)blockIdx.x >= ntiles) return;
+  if (ts.tile >= ntiles) return;
   int ti, tj;
-  tri_decode(blockIdx.x, ti, tj);
+  tri_decode(ts.tile, ti, tj);
   const int i0 = ti * NB, j0 = tj * NB;
   const int n = S.n, d = S.d, tid = threadIdx.x;
 
@@ -322,9 +323,10 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(const Slot* __restrict__ slo
 __global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ slots,
                                                      const SlotParams* __restrict__ params,
                                                      SlotOut* __restrict__ out, int k) {
-  const int b = blockIdx.y;
+  const TileSlot ts = tile_slot();
+  const int b = ts.slot;
   const Slot S = slots[b];
-  const int r = k + 1 + blockIdx.x;
+  const int r = k + 1 + ts.tile;
   if (!slot_live(params[b], out[b]) || r >= S.nblk) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const int i0 = r * NB, k0 = k * NB, ld = S.npad;
@@ -343,13 +345,14 @@ __global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ sl
 __global__ __launch_bounds__(TPB) void k_potrf_update(const Slot* __restrict__ slots,
                                                       const SlotParams* __restrict__ params,
                                                       SlotOut* __restrict__ out, int k) {
-  const int b = blockIdx.y;
+  const TileSlot ts = tile_slot();
+  const int b = ts.slot;
   const Slot S = slots[b];
   if (!slot_live(params[b], out[b])) return;
   const int T = S.nblk - k - 1;  // trailing blocks of this slot (negative past its end)
-  if (T <= 0 || (int)blockIdx.x >= T * (T + 1) / 2) return;
+  if (T <= 0 || ts.tile >= T * (T + 1) / 2) return;
   int ti, tj;
-  tri_decode(blockIdx.x, ti, tj);
+  tri_decode(ts.tile, ti, tj);
   const int i0 = (k + 1 + ti) * NB, j0 = (k + 1 + tj) * NB, k0 = k * NB, ld = S.npad;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   __shared__ double sbuf[4 * KC * LDSS];
@@ -369,12 +372,13 @@ __global__ __launch_bounds__(TPB) void k_potrf_update(const Slot* __restrict__ s
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(TPB) void k_trtri(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
                                                SlotOut* __restrict__ out, int s, int step) {
-  const int b = blockIdx.y;
+  const TileSlot ts = tile_slot();
+  const int b = ts.slot;
   const Slot S = slots[b];
   if (!slot_live(params[b], out[b])) return;
   const int per_pair = s * s;
-  const int pr = blockIdx.x / per_pair;
-  const int rem = blockIdx.x % per_pair;
+  const int pr = ts.tile / per_pair;
+  const int rem = ts.tile % per_pair;
   const int ti = rem / s, tj = rem % s;
   const int top = 2 * pr * s, bot = top + s;
   const int it = bot + ti, jt = top + tj;  // tile row / col of the output
@@ -449,10 +453,11 @@ __device__ inline void lower_matvec_block(const Slot& S, int rb, const double* _
 __global__ __launch_bounds__(TPB) void k_solve_fy(const Slot* __restrict__ slots,
                                                   const SlotParams* __restrict__ params,
                                                   SlotOut* __restrict__ out) {
-  const int b = blockIdx.y;
+  const TileSlot ts = tile_slot();
+  const int b = ts.slot;
   const Slot S = slots[b];
-  if (!slot_live(params[b], out[b]) || (int)blockIdx.x >= S.nblk) return;
-  const int rb = blockIdx.x;
+  if (!slot_live(params[b], out[b]) || ts.tile >= S.nblk) return;
+  const int rb = ts.tile;
   if (S.p > 0) lower_matvec_block(S, rb, S.F, S.n, S.n, S.p, S.V, S.npad, nullptr);
   lower_matvec_block(S, rb, S.y, S.n, S.n, 1, S.V + (size_t)S.p * S.npad, S.npad, nullptr);
 }
@@ -535,27 +540,29 @@ __global__ __launch_bounds__(TPB) void k_gls(const Slot* __restrict__ slots, con
 __global__ __launch_bounds__(TPB) void k_solve_e(const Slot* __restrict__ slots,
                                                  const SlotParams* __restrict__ params,
                                                  SlotOut* __restrict__ out) {
-  const int b = blockIdx.y;
+  const TileSlot ts = tile_slot();
+  const int b = ts.slot;
   const Slot S = slots[b];
-  if (!slot_live(params[b], out[b]) || (int)blockIdx.x >= S.nblk) return;
+  if (!slot_live(params[b], out[b]) || ts.tile >= S.nblk) return;
   const double* r = S.V + (size_t)(S.p + 1) * S.npad;
   double* e = S.V + (size_t)(S.p + 2) * S.npad;
-  lower_matvec_block(S, blockIdx.x, r, S.npad, S.n, 1, e, S.npad, S.red + red_sse_off(S.nblk));
+  lower_matvec_block(S, ts.tile, r, S.npad, S.n, 1, e, S.npad, S.red + red_sse_off(S.nblk));
 }
 
 // x = Linv^T E*, and the 1-norms (max column abs sums) of L and Linv
 __global__ __launch_bounds__(TPB) void k_solve_x(const Slot* __restrict__ slots,
                                                  const SlotParams* __restrict__ params,
                                                  SlotOut* __restrict__ out) {
-  const int b = blockIdx.y;
+  const TileSlot ts = tile_slot();
+  const int b = ts.slot;
   const Slot S = slots[b];
-  if (!slot_live(params[b], out[b]) || (int)blockIdx.x >= S.nblk) return;
+  if (!slot_live(params[b], out[b]) || ts.tile >= S.nblk) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, ld = S.npad, n = S.n;
   const double* e = S.V + (size_t)(S.p + 2) * ld;
   double* x = S.V + (size_t)(S.p + 3) * ld;
   double mW = 0.0, mL = 0.0;
   for (int cc = w; cc < NB; cc += 4) {
-    const int c = blockIdx.x * NB + cc;
+    const int c = ts.tile * NB + cc;
     double s = 0.0, aw = 0.0, al = 0.0;
 #pragma unroll 4
     for (int i = c + lane; i < ld; i += 64) {
@@ -580,8 +587,8 @@ __global__ __launch_bounds__(TPB) void k_solve_x(const Slot* __restrict__ slots,
   }
   __syncthreads();
   if (tid == 0) {
-    S.red[red_colW_off(S.nblk) + blockIdx.x] = fmax(fmax(sm[0][0], sm[0][1]), fmax(sm[0][2], sm[0][3]));
-    S.red[red_colL_off(S.nblk) + blockIdx.x] = fmax(fmax(sm[1][0], sm[1][1]), fmax(sm[1][2], sm[1][3]));
+    S.red[red_colW_off(S.nblk) + ts.tile] = fmax(fmax(sm[0][0], sm[0][1]), fmax(sm[0][2], sm[0][3]));
+    S.red[red_colL_off(S.nblk) + ts.tile] = fmax(fmax(sm[1][0], sm[1][1]), fmax(sm[1][2], sm[1][3]));
   }
 }
 
@@ -658,14 +665,15 @@ template <int DT>
 __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ slots,
                                                        const SlotParams* __restrict__ params,
                                                        SlotOut* __restrict__ out) {
-  const int b = blockIdx.y;
+  const TileSlot ts = tile_slot();
+  const int b = ts.slot;
   const Slot S = slots[b];
   const SlotParams& P = params[b];
   if (!slot_live(P, out[b]) || !P.want_grad) return;
   const int ntiles = S.nblk * (S.nblk + 1) / 2;
-  if ((int)blockIdx.x >= ntiles) return;
+  if (ts.tile >= ntiles) return;
   int it, jt;
-  tri_decode(blockIdx.x, it, jt);
+  tri_decode(ts.tile, it, jt);
   const int ld = S.npad, n = S.n, d = S.d;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const int i0 = it * NB, j0 = jt * NB;
@@ -781,7 +789,7 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
 
   // workgroup reduction of the 2 * DT + 4 partials
   __shared__ double sred[4][2 * DT + 4];
-  double* dst = S.red + red_grad_off(S.nblk) + (size_t)blockIdx.x * GRAD_STRIDE;
+  double* dst = S.red + red_grad_off(S.nblk) + (size_t)ts.tile * GRAD_STRIDE;
 #pragma unroll
   for (int k = 0; k < DT; ++k) {
     const double v1 = wave_sum(s1[k]);
