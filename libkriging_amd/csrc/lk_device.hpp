@@ -13,7 +13,11 @@ namespace lk {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 32;          // K-chunk staged in LDS per step
-constexpr int LDSS = NB + 4;    // LDS row stride (doubles) of a KC x NB staging tile
+// LDS row stride (doubles) of a KC x NB staging tile.  66: the transposed
+// staging (16 threads writing 16 different k rows of one column pair) then
+// falls on 16 distinct even banks (2 * 2 * 66 = 264 = 8 mod 32 words), no
+// conflict; a stride of 68 (= 16 mod 32) put them on 2 banks (8-way).
+constexpr int LDSS = NB + 2;
 constexpr int TPB = 256;        // threads of the tile kernels (4 waves)
 
 // ---------------------------------------------------------------------------

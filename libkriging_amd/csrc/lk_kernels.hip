@@ -339,27 +339,49 @@ __global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ sl
 }
 
 // ---------------------------------------------------------------------------
-// (2c) trailing update: A(i, j) -= L(i, k) L(j, k)^T for k < j <= i
-// grid (T(T+1)/2 with T = nblk_max - k - 1, nslots), 256 threads
+// (2c) deferred update: A(r, c) -= sum_{k in [kb, ke)} L(r, k) L(c, k)^T
+// over tile columns c in [c0, c1] (c1 < 0: every column from c0 on) and
+// rows r >= c.  Two-level blocking: inside a panel of P block columns the
+// update reaches only the panel's own columns (K = 64); the rest of the
+// matrix is updated once per panel with K = 64 P, so the trailing matrix is
+// read and written 1 / P as often.
+// grid (tiles of the widest slot, nslots), 256 threads
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(TPB) void k_potrf_update(const Slot* __restrict__ slots,
                                                       const SlotParams* __restrict__ params,
-                                                      SlotOut* __restrict__ out, int k) {
+                                                      SlotOut* __restrict__ out, int kb, int ke, int c0, int c1) {
   const TileSlot ts = tile_slot();
   const int b = ts.slot;
   const Slot S = slots[b];
   if (!slot_live(params[b], out[b])) return;
-  const int T = S.nblk - k - 1;  // trailing blocks of this slot (negative past its end)
-  if (T <= 0 || ts.tile >= T * (T + 1) / 2) return;
-  int ti, tj;
-  tri_decode(ts.tile, ti, tj);
-  const int i0 = (k + 1 + ti) * NB, j0 = (k + 1 + tj) * NB, k0 = k * NB, ld = S.npad;
+  const int nb = S.nblk;
+  if (c0 >= nb || ke > nb) return;
+  int r, c;
+  if (c1 < 0) {
+    const int T = nb - c0;
+    if (ts.tile >= T * (T + 1) / 2) return;
+    int ti, tj;
+    tri_decode(ts.tile, ti, tj);
+    r = c0 + ti;
+    c = c0 + tj;
+  } else {
+    int t = ts.tile;
+    c = c0;
+    const int cl = min(c1, nb - 1);
+    while (c <= cl && t >= nb - c) {
+      t -= nb - c;
+      ++c;
+    }
+    if (c > cl) return;
+    r = c + t;
+  }
+  const int i0 = r * NB, j0 = c * NB, ld = S.npad;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   __shared__ double sbuf[4 * KC * LDSS];
   d4 acc[2][2];
   double* C = S.A + (size_t)j0 * ld + i0;
   acc_load(acc, C, ld, lane, wr, wc);
-  tile_mainloop<false, false, true>(S.A, ld, i0, S.A, ld, j0, k0, k0 + NB, sbuf, acc, tid, lane, wr, wc);
+  tile_mainloop<false, false, true>(S.A, ld, i0, S.A, ld, j0, kb * NB, ke * NB, sbuf, acc, tid, lane, wr, wc);
   acc_store(acc, C, ld, lane, wr, wc);
 }
 
@@ -1160,10 +1182,18 @@ hipError_t launch_potrf_panel(const Batch& B, int k) {
   hipLaunchKernelGGL(k_potrf_panel, dim3(T, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, k);
   return hipGetLastError();
 }
-hipError_t launch_potrf_update(const Batch& B, int k) {
-  const int T = B.nblk_max - k - 1;
-  if (T <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_potrf_update, dim3(tri(T), B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, k);
+hipError_t launch_potrf_update(const Batch& B, int kb, int ke, int c0, int c1) {
+  const int nb = B.nblk_max;
+  if (c0 >= nb) return hipSuccess;
+  int tiles = 0;
+  if (c1 < 0) {
+    tiles = tri(nb - c0);
+  } else {
+    for (int c = c0; c <= c1 && c < nb; ++c) tiles += nb - c;
+  }
+  if (tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_potrf_update, dim3(tiles, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, kb, ke,
+                     c0, c1);
   return hipGetLastError();
 }
 int trtri_levels(int nblk) {

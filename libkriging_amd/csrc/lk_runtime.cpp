@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 
@@ -19,6 +20,7 @@ Runtime::Runtime(int device) : device_(device) {
   if (device < 0 || device >= count) throw std::runtime_error("no HIP device " + std::to_string(device));
   hip_check(hipSetDevice(device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  if (const char* e = std::getenv("LKGPU_PANEL_BLOCKS")) panel_blocks_ = std::max(1, std::atoi(e));
 }
 
 Runtime::~Runtime() {
@@ -167,10 +169,27 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
 std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_grad) const {
   std::vector<Step> v;
   v.push_back({1u << 0, "build", [B] { return launch_build(B); }});
-  for (int k = 0; k < B.nblk_max; ++k) {
-    v.push_back({1u << 1, "potrf_diag", [B, k] { return launch_potrf_diag(B, k); }});
-    v.push_back({1u << 1, "potrf_panel", [B, k] { return launch_potrf_panel(B, k); }});
-    v.push_back({(1u << 1) | (k == 0 ? 1u << 2 : 0u), "potrf_update", [B, k] { return launch_potrf_update(B, k); }});
+  // blocked right-looking Cholesky, two levels: panels of P block columns;
+  // inside a panel each step updates only the panel's remaining columns,
+  // then one trailing update with K = 64 P per panel
+  const int P = panel_blocks_;
+  bool first_update = true;
+  for (int K0 = 0; K0 < B.nblk_max; K0 += P) {
+    const int K1 = std::min(K0 + P, B.nblk_max);
+    for (int k = K0; k < K1; ++k) {
+      v.push_back({1u << 1, "potrf_diag", [B, k] { return launch_potrf_diag(B, k); }});
+      v.push_back({1u << 1, "potrf_panel", [B, k] { return launch_potrf_panel(B, k); }});
+      if (k + 1 < K1) {
+        v.push_back({(1u << 1) | (first_update ? 1u << 2 : 0u), "potrf_update (panel)",
+                     [B, k, K1] { return launch_potrf_update(B, k, k + 1, k + 1, K1 - 1); }});
+        first_update = false;
+      }
+    }
+    if (K1 < B.nblk_max) {
+      v.push_back({(1u << 1) | (first_update ? 1u << 2 : 0u), "potrf_update (trailing)",
+                   [B, K0, K1] { return launch_potrf_update(B, K0, K1, K1, -1); }});
+      first_update = false;
+    }
   }
   const int lv = trtri_levels(B.nblk_max);
   for (int l = 0; l < lv; ++l) {

@@ -161,6 +161,8 @@ class Runtime {
   };
   GraphEntry graphs_[2];
   int timed_stage_ = -1;
+  // block columns per Cholesky panel (two-level blocking); LKGPU_PANEL_BLOCKS
+  int panel_blocks_ = 4;
   hipEvent_t tev_[2] = {nullptr, nullptr};
   double timed_sum_ms_ = 0.0;
   int timed_count_ = 0;
