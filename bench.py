@@ -58,6 +58,22 @@ def cpu_baseline(X, y, theta, gpu_ll, gpu_grad, budget_s=15.0, max_evals=20):
                       f"single-threaded C oracle"}, ll_err, g_err
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batch64_n1024_d6.txt")
+
+
+def lauum_traffic_bytes():
+    """HBM bytes per k_lauum_adjoint launch of this workload, from the
+    copy-calibrated FETCH_SIZE / WRITE_SIZE passes summarised in profiles/"""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("k_lauum_adjoint traffic_bytes_per_launch"):
+                    return int(line.split()[-1])
+    except OSError:
+        pass
+    return None
+
+
 def cholesky_large(N, n=8192, d=10, reps=3):
     """The single-evaluation Cholesky at n = 8192 (gauss kernel, d = 10,
     theta log-U[0.3, 1.5], nugget ratio 1e-6), timed live over its own
@@ -82,6 +98,70 @@ def cholesky_large(N, n=8192, d=10, reps=3):
             "frac_of_fp64_mfma_peak": tflops / MFMA_FP64_PEAK_TFLOPS}
 
 
+def nested_workload(seed=127, n=200_000, d=5, groups=256):
+    """The uneven NestedKriging configuration: X ~ U[0,1)^(n x d),
+    y = sum_j sin(4 x_j) + x_1 x_5 + N(0, 1e-4); 256 group sizes drawn
+    lognormal (sigma_log 0.5 around n / 256), clipped to [300, 1500] and
+    adjusted by +-1 until they sum to n; points assigned by a seeded
+    permutation."""
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(size=(n, d))
+    y = np.sum(np.sin(4.0 * X), axis=1) + X[:, 0] * X[:, d - 1] + rng.normal(0.0, 1e-2, size=n)
+    sizes = np.clip(np.round(rng.lognormal(np.log(n / groups), 0.5, size=groups)), 300, 1500).astype(np.int64)
+    while sizes.sum() != n:
+        k = rng.integers(groups)
+        step = 1 if sizes.sum() < n else -1
+        if 300 <= sizes[k] + step <= 1500:
+            sizes[k] += step
+    perm = rng.permutation(n)
+    group_of = np.empty(n, dtype=np.int32)
+    o = 0
+    for g, sz in enumerate(sizes):
+        group_of[perm[o:o + sz]] = g
+        o += sz
+    Xn = rng.uniform(size=(10_000, d))
+    return X, y, group_of, sizes, Xn
+
+
+def run_nested(args):
+    """One NestedKriging fit (all 256 groups' BFGS in shared batched
+    pipelines) and one NK predict; value = predicted points per second."""
+    import libkriging_amd as lk
+    X, y, group_of, sizes, Xn = nested_workload()
+    q = args.nested_points
+    Xq = Xn[:q]
+    nk = lk.NestedKriging("matern3_2")
+    t0 = time.perf_counter()
+    nk.fit_groups(y, X, group_of, "constant", "BFGS")
+    t_fit = time.perf_counter() - t0
+    fit_evals, fit_batches = nk.fit_statistics()
+    nk.predict(Xq[:64], True)  # warm
+    t0 = time.perf_counter()
+    mean, sd = nk.predict(Xq, True)
+    t_pred = time.perf_counter() - t0
+    s = sizes.astype(np.float64)
+    pair_flops = (s.sum() ** 2 - np.sum(s ** 2)) * q  # 2 sum_{i<j} n_i n_j per point
+    achieved = pair_flops / t_pred / 1e12
+    line = {
+        "metric": "NestedKriging NK predict points/s (N=200000, 256 uneven groups, d=5, matern3_2)",
+        "value": q / t_pred, "unit": "points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * t_pred, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "NestedKriging uneven groups: fit (BFGS per group) + NK predict",
+                   "n": int(s.sum()), "d": 5, "groups": len(sizes), "group_min": int(sizes.min()),
+                   "group_max": int(sizes.max()), "points": q, "kernel": "matern3_2"},
+        "fit_s": t_fit, "fit_evaluations": int(fit_evals), "fit_batches": int(fit_batches),
+        "fit_evals_per_s": fit_evals / t_fit,
+        "predict_fallbacks": nk.predict_fallbacks(),
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / MFMA_FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_nk_pair",
+                     "note": "pair-contraction flops over the whole predict call (wall clock)"},
+        "theta": nk.theta().tolist(), "sigma2": nk.sigma2(), "beta0": nk.beta0(),
+        "finite": bool(np.all(np.isfinite(mean)) and np.all(np.isfinite(sd))),
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +170,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chol", action="store_true", help="skip the n = 8192 Cholesky measurement")
+    ap.add_argument("--workload", default="multistart", choices=["multistart", "nested"],
+                    help="multistart: the headline batched LL+grad; nested: NestedKriging fit + NK predict")
+    ap.add_argument("--nested-points", type=int, default=2000, help="prediction points of the nested workload")
     ap.add_argument("--stage", default="adjoint", help="kernel timed live for the roofline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help=f"also run a {COPY_BYTES >> 20} MiB device copy (counter calibration)")
@@ -98,6 +181,10 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the multistart batch would shard by starts, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "nested":
+        run_nested(args)
+        return
 
     from libkriging_amd import _native as N
 
@@ -143,7 +230,8 @@ def main():
         flops = NSTART * n ** 3 / 3.0
         achieved = flops / (stage_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / MFMA_FP64_PEAK_TFLOPS, "traffic": None,
+                "frac": achieved / MFMA_FP64_PEAK_TFLOPS, "traffic": lauum_traffic_bytes(),
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated (profiles/)",
                 "kernel": "k_lauum_adjoint", "avg_ms": stage_ms, "launches": stage_count}
     elif args.stage == "build":
         nbytes = NSTART * (8.0 * n * (n + 1) / 2 + 8.0 * n * d)
@@ -175,6 +263,8 @@ def main():
         "plain_equals_batched": same,
         "roofline": roof,
     }
+    copy_ms = ctx.device_copy(COPY_BYTES, 5)
+    line["hbm_copy_GBps"] = 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9  # read + write of a plain copy kernel
     if not args.no_chol:
         line["cholesky_n8192"] = cholesky_large(N)
     if not args.no_cpu:
@@ -186,13 +276,9 @@ def main():
         # a plain device copy of known size, run in the same process so that
         # a counter-collection pass over this command also measures it:
         # FETCH_SIZE / WRITE_SIZE of this kernel calibrate those counters
-        import torch
-        src = torch.empty(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes": COPY_BYTES}), file=sys.stderr)
+        ms = ctx.device_copy(COPY_BYTES, 3)  # k_copy: reads and writes COPY_BYTES per launch
+        print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "copy_ms": ms,
+                          "copy_GBps": 2 * COPY_BYTES / (ms * 1e-3) / 1e9}), file=sys.stderr)
 
 
 if __name__ == "__main__":

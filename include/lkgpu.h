@@ -23,6 +23,8 @@
 #ifndef LKGPU_H
 #define LKGPU_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -120,6 +122,12 @@ int lkgpu_synchronize(lkgpu_context* ctx);
 
 /* Use (1) or bypass (0) hipGraph replay of the evaluation pipeline. */
 int lkgpu_set_graph(lkgpu_context* ctx, int enable);
+
+/* A plain device-to-device copy kernel over `bytes` (a multiple of 16),
+ * repeated `reps` times; *ms_out = mean time per copy.  No counterpart in the
+ * reference: the HBM bandwidth reference and the calibration of the
+ * FETCH_SIZE / WRITE_SIZE counters behind the roofline traffic figures. */
+int lkgpu_device_copy(lkgpu_context* ctx, size_t bytes, int reps, double* ms_out);
 
 /* ------------------------------------------------------------------------ */
 /* Layer 2: the Kriging object (libkriging_c.h names and conventions)       */
@@ -264,6 +272,9 @@ int lk_nested_kriging_groups(void* ptr, int* group_of_out);
 void* lk_nested_kriging_submodel(void* ptr, int i);
 int lk_nested_kriging_set_predict_chunk(void* ptr, int chunk);
 int lk_nested_kriging_predict_fallbacks(void* ptr);
+/* LL+grad evaluations and batched pipelines of the groups' optimisation in
+ * the last fit (all groups share each pipeline); no reference counterpart. */
+int lk_nested_kriging_fit_statistics(void* ptr, int* evaluations, int* batches);
 
 #ifdef __cplusplus
 }

@@ -123,6 +123,41 @@ int lkgpu_synchronize(lkgpu_context* ctx) {
   });
 }
 
+int lkgpu_device_copy(lkgpu_context* ctx, size_t bytes, int reps, double* ms_out) {
+  return guarded([&] {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    if (bytes == 0 || bytes % 16 != 0 || reps <= 0) throw std::invalid_argument("bytes must be a positive multiple of 16");
+    lk::hip_check(hipSetDevice(ctx->rt.device()), "hipSetDevice");
+    void *src = nullptr, *dst = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+      if (src) hipFree(src);
+      if (dst) hipFree(dst);
+      if (e0) hipEventDestroy(e0);
+      if (e1) hipEventDestroy(e1);
+    };
+    try {
+      lk::hip_check(hipMalloc(&src, bytes), "hipMalloc");
+      lk::hip_check(hipMalloc(&dst, bytes), "hipMalloc");
+      lk::hip_check(hipEventCreate(&e0), "event");
+      lk::hip_check(hipEventCreate(&e1), "event");
+      hipStream_t st = ctx->rt.stream();
+      lk::hip_check(lk::launch_copy(src, dst, bytes, st), "copy");  // warm
+      lk::hip_check(hipEventRecord(e0, st), "event");
+      for (int r = 0; r < reps; ++r) lk::hip_check(lk::launch_copy(dst, src, bytes, st), "copy");
+      lk::hip_check(hipEventRecord(e1, st), "event");
+      lk::hip_check(hipEventSynchronize(e1), "event sync");
+      float ms = 0.f;
+      lk::hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+      if (ms_out) *ms_out = ms / reps;
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
 int lkgpu_set_timed_stage(lkgpu_context* ctx, int stage) {
   return guarded([&] {
     if (ctx == nullptr) throw std::invalid_argument("context is NULL");

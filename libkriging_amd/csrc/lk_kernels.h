@@ -128,6 +128,8 @@ struct PredictArgs {
   int kernel, linear_trend;
 };
 hipError_t launch_predict(const PredictArgs& a, hipStream_t stream);
+// plain device copy of `bytes` (a multiple of 16)
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // NestedKriging "NK" aggregation (NestedKriging::predict_nk,

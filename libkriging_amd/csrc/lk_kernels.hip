@@ -431,8 +431,11 @@ __global__ __launch_bounds__(TPB) void k_trtri(const Slot* __restrict__ slots, c
 // ---------------------------------------------------------------------------
 // out[:, c] = Linv * in[:, c] for rows of block rb (grid (nblk_max, nslots))
 // in columns come from `src` (n rows, ld = n, zero beyond n) ; out to V.
+// Column c of the input is src[c * lds + k] for c < ncols_src and src2[k]
+// for c == ncols_src (when src2 is given), so [F | y] go through in one pass.
 __device__ inline void lower_matvec_block(const Slot& S, int rb, const double* __restrict__ src, int lds, int nsrc,
-                                          int ncols, double* __restrict__ dst, int ldd, double* __restrict__ part_sq) {
+                                          int ncols, double* __restrict__ dst, int ldd, double* __restrict__ part_sq,
+                                          const double* __restrict__ src2 = nullptr) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i0 = rb * NB, ld = S.npad, n = S.n;
   const int kend = min(i0 + NB, n);  // columns k <= row, and k < n (src is zero beyond n)
@@ -446,7 +449,10 @@ __device__ inline void lower_matvec_block(const Slot& S, int rb, const double* _
       const double lv = S.W[(size_t)k * ld + i0 + lane];
 #pragma unroll
       for (int c = 0; c < 4; ++c)
-        if (c < nc) acc[c] = fma(lv, src[(size_t)(c0 + c) * lds + k], acc[c]);
+        if (c < nc) {
+          const double* col = (src2 != nullptr && c0 + c == ncols - 1) ? src2 : src + (size_t)(c0 + c) * lds;
+          acc[c] = fma(lv, col[k], acc[c]);
+        }
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) red[w][c][lane] = acc[c];
@@ -480,8 +486,8 @@ __global__ __launch_bounds__(TPB) void k_solve_fy(const Slot* __restrict__ slots
   const Slot S = slots[b];
   if (!slot_live(params[b], out[b]) || ts.tile >= S.nblk) return;
   const int rb = ts.tile;
-  if (S.p > 0) lower_matvec_block(S, rb, S.F, S.n, S.n, S.p, S.V, S.npad, nullptr);
-  lower_matvec_block(S, rb, S.y, S.n, S.n, 1, S.V + (size_t)S.p * S.npad, S.npad, nullptr);
+  // [F | y] -> V columns 0 .. p in one pass over this row block of Linv
+  lower_matvec_block(S, rb, S.F, S.n, S.n, S.p + 1, S.V, S.npad, nullptr, S.y);
 }
 
 // GLS: R* = chol_upper(F*'F*), beta = R*^-1 R*^-T F*'y*, r = y - F beta
@@ -1139,6 +1145,18 @@ __global__ __launch_bounds__(TPB) void k_predict_reduce(PredictArgs a) {
   if (isnan(v) || v < 0.0) v = 0.0;
   a.out[j] = mean;
   a.out[a.m + j] = v;
+}
+
+// plain device copy (16-byte vectors, grid-stride): the HBM bandwidth and
+// traffic-counter reference for the roofline figures
+__global__ __launch_bounds__(TPB) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(TPB), 0, stream, (const double2*)src, (double2*)dst, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_predict(const PredictArgs& a, hipStream_t stream) {

@@ -267,6 +267,12 @@ void NestedKriging::fit_impl(RegModel regmodel, const std::string& optim, const 
     jp.push_back(jobs.back().get());
   }
   Kriging::run_fit_jobs(jp);
+  fit_evals_ = 0;
+  fit_batches_ = 0;
+  for (const auto& k : sub_) {
+    fit_evals_ += k->fit_evaluations();
+    fit_batches_ = std::max(fit_batches_, k->fit_batches());
+  }
   rt_ = sub_[0]->runtime();
   unify(objective);
   if (aggregation_ == Aggregation::NK) precompute_nk();

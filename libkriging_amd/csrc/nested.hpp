@@ -60,6 +60,9 @@ class NestedKriging {
   int n() const { return n_; }
   int d() const { return d_; }
   int predict_fallbacks() const { return fallbacks_; }  // points solved by LU / pinv in the last predict
+  // evaluations and batched pipelines of the groups' optimisation in the last fit
+  int fit_evaluations() const { return fit_evals_; }
+  int fit_batches() const { return fit_batches_; }
   std::string summary() const;
 
  private:
@@ -88,6 +91,7 @@ class NestedKriging {
   double sigma2_ = 0.0, beta0_ = 0.0;
   bool fitted_ = false;
   int fallbacks_ = 0;
+  int fit_evals_ = 0, fit_batches_ = 0;
 
   // NK device state (group-ordered, padded to 64-row tiles)
   int device_ = 0;

@@ -219,6 +219,14 @@ int lk_nested_kriging_set_predict_chunk(void* ptr, int chunk) {
   return guarded([&] { NK(ptr)->set_predict_chunk(chunk); });
 }
 
+int lk_nested_kriging_fit_statistics(void* ptr, int* evaluations, int* batches) {
+  return guarded([&] {
+    lk::NestedKriging* k = NK(ptr);
+    if (evaluations) *evaluations = k->fit_evaluations();
+    if (batches) *batches = k->fit_batches();
+  });
+}
+
 int lk_nested_kriging_predict_fallbacks(void* ptr) {
   int v = -1;
   guarded([&] { v = NK(ptr)->predict_fallbacks(); });

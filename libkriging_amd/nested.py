@@ -40,6 +40,7 @@ N.register("lk_nested_kriging_groups", _i, [_vp, _ip])
 N.register("lk_nested_kriging_submodel", _vp, [_vp, _i])
 N.register("lk_nested_kriging_set_predict_chunk", _i, [_vp, _i])
 N.register("lk_nested_kriging_predict_fallbacks", _i, [_vp])
+N.register("lk_nested_kriging_fit_statistics", _i, [_vp, _ip, _ip])
 
 
 def _nparams(parameters, d):
@@ -196,6 +197,12 @@ class NestedKriging:
 
     def set_warp_subsample(self, m):
         raise ValueError("warping (WarpKriging submodels) is not supported on this path")
+
+    def fit_statistics(self):
+        """(evaluations, batched pipelines) of the groups' optimisation in the last fit"""
+        e, b = ctypes.c_int(), ctypes.c_int()
+        N.check(N.lib().lk_nested_kriging_fit_statistics(self._h, ctypes.byref(e), ctypes.byref(b)))
+        return e.value, b.value
 
     def predict_fallbacks(self):
         return N.lib().lk_nested_kriging_predict_fallbacks(self._h)
