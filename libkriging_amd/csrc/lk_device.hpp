@@ -12,7 +12,7 @@ namespace lk {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int KC = 32;          // K-chunk staged in LDS per step
+constexpr int KC = 16;          // K-chunk staged in LDS per step
 // LDS row stride (doubles) of a KC x NB staging tile.  66: the transposed
 // staging (16 threads writing 16 different k rows of one column pair) then
 // falls on 16 distinct even banks (2 * 2 * 66 = 264 = 8 mod 32 words), no
@@ -110,22 +110,29 @@ __device__ inline void mma_chunk(const double* __restrict__ sA, const double* __
 // Register-staged copy of one KC x NB chunk: the global loads of the next
 // chunk are issued before the MFMAs of the current one and written to the
 // other LDS buffer afterwards (software pipelining over K).
+constexpr int CHUNK_V2 = KC * NB / (2 * TPB);  // double2 per thread per KC x NB chunk
+static_assert(KC % 8 == 0 && CHUNK_V2 >= 1, "chunk layout");
 struct ChunkRegs {
-  double2 v[4];
+  double2 v[CHUNK_V2];
 };
 
+// TRANS = false: a thread copies pairs of rows (ii, ii + 1) of k rows
+//   tid / 32 + 8 q; TRANS = true: pairs of k (kk, kk + 1) of rows
+//   tid / (KC / 2) + (2 TPB / KC) q.  Both read 16-byte vectors along the
+//   contiguous dimension of memory.
 template <bool TRANS>
 __device__ inline void chunk_load(ChunkRegs& r, const double* __restrict__ M, int ld, int i0, int k0, int tid) {
   if (!TRANS) {
     const int ii = 2 * (tid & 31);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+    for (int q = 0; q < CHUNK_V2; ++q)
       r.v[q] = *reinterpret_cast<const double2*>(M + (size_t)(k0 + (tid >> 5) + 8 * q) * ld + i0 + ii);
   } else {
-    const int kk = 2 * (tid & 15);
+    constexpr int H = KC / 2;
+    const int kk = 2 * (tid % H);
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-      r.v[q] = *reinterpret_cast<const double2*>(M + (size_t)(i0 + (tid >> 4) + 16 * q) * ld + k0 + kk);
+    for (int q = 0; q < CHUNK_V2; ++q)
+      r.v[q] = *reinterpret_cast<const double2*>(M + (size_t)(i0 + tid / H + (TPB / H) * q) * ld + k0 + kk);
   }
 }
 
@@ -134,16 +141,17 @@ __device__ inline void chunk_store(const ChunkRegs& r, double* __restrict__ dst,
   if (!TRANS) {
     const int ii = 2 * (tid & 31);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < CHUNK_V2; ++q) {
       const int kk = (tid >> 5) + 8 * q;
       dst[kk * LDSS + ii] = r.v[q].x;
       dst[kk * LDSS + ii + 1] = r.v[q].y;
     }
   } else {
-    const int kk = 2 * (tid & 15);
+    constexpr int H = KC / 2;
+    const int kk = 2 * (tid % H);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int ii = (tid >> 4) + 16 * q;
+    for (int q = 0; q < CHUNK_V2; ++q) {
+      const int ii = tid / H + (TPB / H) * q;
       dst[kk * LDSS + ii] = r.v[q].x;
       dst[(kk + 1) * LDSS + ii] = r.v[q].y;
     }

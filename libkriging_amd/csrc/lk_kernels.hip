@@ -712,8 +712,12 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
   tile_mainloop<true, true, false>(S.W, ld, i0, S.W, ld, j0, it * NB, S.nblk * NB, sbuf, acc, tid, lane, wr, wc);
 
   // epilogue: X rows of both blocks and the x vector in LDS (reuse sbuf)
-  double* sXi = sbuf;  // d x NB
-  double* sXj = sbuf + DMAX * NB;
+  // the X rows need 2 DT NB doubles: in the staging buffer when it is large
+  // enough (it is idle now), else in an array of their own
+  constexpr bool kReuse = 4 * KC * LDSS >= 2 * DT * NB;
+  __shared__ double sXown[kReuse ? 1 : 2 * DT * NB];
+  double* sXi = kReuse ? sbuf : sXown;  // d x NB
+  double* sXj = sXi + DT * NB;
   __shared__ double sxi[NB], sxj[NB];
   const double* xv = S.V + (size_t)(S.p + 3) * ld;
   for (int q = tid; q < d * NB; q += TPB) {
