@@ -86,12 +86,12 @@ __device__ inline bool slot_live(const SlotParams& P, const SlotOut& O) {
 // op(M)(i0 + ii, k0 + kk):
 //   TRANS = false: op(M)(i, k) = M[k * ld + i]   (i contiguous in memory)
 //   TRANS = true : op(M)(i, k) = M[i * ld + k]   (k contiguous in memory)
-template <bool NEG>
+template <bool NEG, int KCH = KC>
 __device__ inline void mma_chunk(const double* __restrict__ sA, const double* __restrict__ sB, d4 (&acc)[2][2],
                                  int lane, int wr, int wc) {
   const int r = lane & 15, kq = lane >> 4;
 #pragma unroll
-  for (int k0 = 0; k0 < KC; k0 += 4) {
+  for (int k0 = 0; k0 < KCH; k0 += 4) {
     const double* pa = sA + (k0 + kq) * LDSS + wr * 32 + r;
     const double* pb = sB + (k0 + kq) * LDSS + wc * 32 + r;
     double a0 = pa[0], a1 = pa[16];

@@ -142,16 +142,20 @@ __global__ __launch_bounds__(TPB) void k_nk_alpha(NkArgs a, const double* __rest
 // then one fixed-order reduction over the 16 row lanes and the two row waves.
 // ---------------------------------------------------------------------------
 template <int DT>
-__global__ __launch_bounds__(TPB) void k_nk_pair(NkArgs a) {
+__global__ __launch_bounds__(TPB, 2) void k_nk_pair(NkArgs a) {
   const int2 pr = a.pairs[blockIdx.x];
   const int t0 = blockIdx.y * NB;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const NkGroup Gi = a.groups[pr.x], Gj = a.groups[pr.y];
   const int d = a.d, ld = a.Npad;
-  __shared__ double sA[KC * LDSS];   // R chunk: sA[kk * LDSS + ii] = R(a_ii, b_kk)
-  __shared__ double sB[KC * LDSS];   // W_j chunk: sB[kk * LDSS + jj] = W_j(b_kk, t0 + jj)
-  __shared__ double sXa[DMAX * NB];  // rows of the current a tile
-  __shared__ double sXb[DMAX * KC];  // rows of the current b chunk
+  // deeper chunks than the plain tile kernels: every generated R value is
+  // reused by 64 points, so fewer, longer chunks amortise the barriers
+  constexpr int PKC = 32;
+  constexpr int WPT = PKC * NB / TPB;  // W_j values per thread per chunk
+  __shared__ double sA[PKC * LDSS];    // R chunk: sA[kk * LDSS + ii] = R(a_ii, b_kk)
+  __shared__ double sB[PKC * LDSS];    // W_j chunk: sB[kk * LDSS + jj] = W_j(b_kk, t0 + jj)
+  __shared__ double sXa[DMAX * NB];    // rows of the current a tile
+  __shared__ double sXb[DMAX * PKC];   // rows of the current b chunk
   __shared__ double sred[2][NB];
   double it[DT];
 #pragma unroll
@@ -165,6 +169,22 @@ __global__ __launch_bounds__(TPB) void k_nk_pair(NkArgs a) {
   const double* Wi = a.C + (size_t)t0 * ld + Gi.off;
   const double* Wj = a.C + (size_t)t0 * ld + Gj.off;
   const int ii = tid & 63;
+  // register stage of the next chunk: W_j values and (for tid < d PKC / TPB
+  // rounds) X_j coordinates, loaded while the current chunk is computed
+  double rw[WPT];
+  double rx[(DMAX * PKC) / TPB];
+  auto load_chunk = [&](int b0) {
+#pragma unroll
+    for (int r = 0; r < WPT; ++r) {
+      const int q = tid + r * TPB;
+      rw[r] = Wj[(size_t)(q / PKC) * ld + b0 + (q % PKC)];
+    }
+#pragma unroll
+    for (int r = 0; r < (DMAX * PKC) / TPB; ++r) {
+      const int q = tid + r * TPB;
+      rx[r] = (q < d * PKC) ? a.Xg[(size_t)(q / PKC) * ld + Gj.off + b0 + (q % PKC)] : 0.0;
+    }
+  };
   for (int a0 = 0; a0 < Gi.npad; a0 += NB) {
     for (int q = tid; q < d * NB; q += TPB) {
       const int k = q / NB, r = q % NB;
@@ -172,29 +192,30 @@ __global__ __launch_bounds__(TPB) void k_nk_pair(NkArgs a) {
     }
     d4 acc[2][2];
     acc_zero(acc);
-    for (int b0 = 0; b0 < Gj.npad; b0 += KC) {
-      for (int q = tid; q < d * KC; q += TPB) {
-        const int k = q / KC, r = q % KC;
-        sXb[q] = a.Xg[(size_t)k * ld + Gj.off + b0 + r];
-      }
-      // W_j chunk: KC rows x 64 points, rows contiguous in memory
+    load_chunk(0);
+    for (int b0 = 0; b0 < Gj.npad; b0 += PKC) {
 #pragma unroll
-      for (int r = 0; r < (KC * NB) / TPB; ++r) {
+      for (int r = 0; r < WPT; ++r) {
         const int q = tid + r * TPB;
-        const int kk = q % KC, jj = q / KC;
-        sB[kk * LDSS + jj] = Wj[(size_t)jj * ld + b0 + kk];
+        sB[(q % PKC) * LDSS + q / PKC] = rw[r];
+      }
+#pragma unroll
+      for (int r = 0; r < (DMAX * PKC) / TPB; ++r) {
+        const int q = tid + r * TPB;
+        if (q < d * PKC) sXb[q] = rx[r];
       }
       __syncthreads();
+      if (b0 + PKC < Gj.npad) load_chunk(b0 + PKC);
       double dx[DT];
 #pragma unroll
-      for (int r = 0; r < (KC * NB) / TPB; ++r) {
+      for (int r = 0; r < (PKC * NB) / TPB; ++r) {
         const int kk = (tid >> 6) + 4 * r;
 #pragma unroll
-        for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? sXa[k * NB + ii] - sXb[k * KC + kk] : 0.0;
+        for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? sXa[k * NB + ii] - sXb[k * PKC + kk] : 0.0;
         sA[kk * LDSS + ii] = kern_rho<DT>(a.kernel, dx, it);
       }
       __syncthreads();
-      mma_chunk<false>(sA, sB, acc, lane, wr, wc);
+      mma_chunk<false, PKC>(sA, sB, acc, lane, wr, wc);
       __syncthreads();
     }
     // epilogue: weight by W_i(a, t) and sum this tile's rows into part
