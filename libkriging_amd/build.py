@@ -17,6 +17,7 @@ LIB = os.path.join(HERE, "liblkgpu.so")
 SOURCES = [
     "lk_kernels.hip",
     "lk_nested.hip",
+    "lk_rinv.hip",
     "lk_runtime.cpp",
     "lk_capi.cpp",
     "lbfgsb.cpp",

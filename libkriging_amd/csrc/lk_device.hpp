@@ -282,4 +282,57 @@ __device__ inline double kern_rho(int kern, const double* dx, const double* it) 
   }
 }
 
+// rho(dx) and g_k = d ln rho / d theta_k (Covariance.cpp:38-179); it = 1 / theta,
+// it2 = it^2, entries k >= d hold dx = 0 (their g is 0)
+template <int DT>
+__device__ inline double kern_rho_g(int kern, const double* dx, const double* it, const double* it2, double* g) {
+  double s = 0.0, pr = 1.0;
+  if (kern == K_GAUSS) {
+#pragma unroll
+    for (int k = 0; k < DT; ++k) {
+      const double v = dx[k] * it[k];
+      s = fma(v, v, s);
+      g[k] = (dx[k] * dx[k]) * (it2[k] * it[k]);
+    }
+    return exp(-0.5 * s);
+  } else if (kern == K_EXP) {
+#pragma unroll
+    for (int k = 0; k < DT; ++k) {
+      s += fabs(dx[k] * it[k]);
+      g[k] = fabs(dx[k]) * it2[k];
+    }
+    return exp(-s);
+  } else if (kern == K_MATERN32) {
+#pragma unroll
+    for (int k = 0; k < DT; ++k) {
+      const double u = SQRT3 * fabs(dx[k] * it[k]);
+      s += u;
+      pr *= 1.0 + u;
+      g[k] = (u * u) / (1.0 + u) * it[k];
+    }
+    return exp(-s) * pr;
+  } else {
+#pragma unroll
+    for (int k = 0; k < DT; ++k) {
+      const double u = SQRT5 * fabs(dx[k] * it[k]);
+      const double aa = 1.0 + u;
+      const double bb = (u * u) * (1.0 / 3.0);
+      s += u;
+      pr *= aa + bb;
+      g[k] = (aa * bb) / (aa + bb) * it[k];
+    }
+    return exp(-s) * pr;
+  }
+}
+
+// sum of a per-thread value over the 256 threads of a tile workgroup, in a
+// fixed order (wave butterflies, then the 4 wave sums in index order)
+__device__ inline double block_sum4(double v, double* sh4, int lane, int w) {
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) sh4[w] = v;
+  __syncthreads();
+  return (sh4[0] + sh4[1]) + (sh4[2] + sh4[3]);
+}
+
 }  // namespace lk

@@ -625,7 +625,7 @@ __global__ __launch_bounds__(TPB) void k_solve_x(const Slot* __restrict__ slots,
 // LinearAlgebra.cpp:68-115).  One thread per slot.
 // ---------------------------------------------------------------------------
 __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
-                              SlotOut* __restrict__ out, int nslots) {
+                              SlotOut* __restrict__ out, int nslots, int rinv_path) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nslots) return;
   const Slot S = slots[b];
@@ -640,6 +640,9 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
     nL = fmax(nL, S.red[red_colL_off(nb) + q]);
     nW = fmax(nW, S.red[red_colW_off(nb) + q]);
   }
+  // gradient path: red_colW holds max Z_ii = max |L^-1 e_i|_2^2 per block,
+  // and |L^-1|_1 <= sqrt(n) max_i |L^-1 e_i|_2 bounds the exact norm
+  if (rinv_path) nW = sqrt((double)S.n * nW);
   const double n = (double)S.n;
   const double two_pi = 2.0 * 3.14159265358979323846;
   O.logdet = logdet;
@@ -648,7 +651,8 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
   O.norm1_Linv = nW;
   const double rc = (1.0 / nL) / nW;
   O.rcond2 = rc * rc;
-  if (P.rcond_check && !(O.rcond2 >= P.min_rcond) && O.status == ST_OK) O.status = ST_RCOND_SUSPECT;
+  if (P.rcond_check && !(O.rcond2 >= P.min_rcond) && O.status == ST_OK)
+    O.status = rinv_path ? ST_RCOND_BOUND : ST_RCOND_SUSPECT;
   double ll, s2g;
   if (S.noise == NOISE_NUGGET) {
     const double a = P.alpha;
@@ -1246,9 +1250,9 @@ hipError_t launch_solve_x(const Batch& B) {
   hipLaunchKernelGGL(k_solve_x, dim3(B.nblk_max, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   return hipGetLastError();
 }
-hipError_t launch_finalize_ll(const Batch& B) {
+hipError_t launch_finalize_ll(const Batch& B, bool rinv_path) {
   hipLaunchKernelGGL(k_finalize_ll, dim3((B.nslots + 63) / 64), dim3(64), 0, B.stream, B.slots, B.params, B.out,
-                     B.nslots);
+                     B.nslots, rinv_path ? 1 : 0);
   return hipGetLastError();
 }
 hipError_t launch_lauum_adjoint(const Batch& B) {

@@ -109,6 +109,9 @@ class Runtime {
 
   CholConfig chol;
   bool use_graph = true;
+  void set_gradient_path(int path) { rinv_path_ = path != 0; }
+  void recheck_rcond_bound(int ns, BenchMap* bench);
+  int gradient_path() const { return rinv_path_ ? 1 : 0; }
 
   // Live timing of one pipeline stage with a HIP event pair recorded on the
   // pipeline stream (inside the captured graph when graphs are used).
@@ -163,6 +166,12 @@ class Runtime {
   int timed_stage_ = -1;
   // block columns per Cholesky panel (two-level blocking); LKGPU_PANEL_BLOCKS
   int panel_blocks_ = 4;
+  // gradient evaluations: 0 = L^-1 (trtri) + R^-1 = L^-T L^-1 (lauum,
+  // default), 1 = R^-1 sweep without L^-1; LKGPU_GRAD_PATH or set_gradient_path
+  bool rinv_path_ = false;
+  // kind of the last run on the slots: L^-1, F*, y*, E* are formed only by
+  // evaluations that do not take the R^-1 sweep
+  bool last_rinv_ = false;
   hipEvent_t tev_[2] = {nullptr, nullptr};
   double timed_sum_ms_ = 0.0;
   int timed_count_ = 0;
