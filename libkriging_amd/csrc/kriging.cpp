@@ -670,20 +670,31 @@ void Kriging::run_fit_jobs(const std::vector<FitJob*>& jobs_in) {
 // predict (Kriging::predict, Kriging.cpp:2241-2285 -> predict_impl,
 // KrigingImpl.cpp:145-243): mean and standard deviation at m points
 // ---------------------------------------------------------------------------
-void Kriging::predict(const double* Xn, int m, bool want_stdev, double* mean, double* stdev) {
+void Kriging::predict(const double* Xn, int m, bool want_stdev, double* mean, double* stdev, bool want_cov,
+                      double* cov, bool want_deriv, double* mean_deriv, double* stdev_deriv) {
   if (is_empty_) throw std::runtime_error("model is not fitted");
   const int d = d_, p = p_;
   if (m <= 0) return;
   hip_check(hipSetDevice(device_), "hipSetDevice");
-  const int chunk = std::min(m, 8192);
+  // points per chunk: the covariance needs every pair in one chunk (m x m,
+  // 8 GiB at the 32768-point limit); the derivatives carry d extra
+  // npad x chunk matrices (bounded to ~4 GiB)
+  int chunk = std::min(m, 8192);
+  if (want_cov) {
+    if (m > 32768) throw std::invalid_argument("predict: return_cov is limited to 32768 points per call on this path");
+    chunk = m;
+  }
+  if (want_deriv) {
+    const double per_col = 16.0 * npad_ * d;
+    const int cmax = std::max(64, (int)std::min<double>(8192.0, (4.0 * (1ull << 30)) / per_col) / 64 * 64);
+    if (!want_cov) chunk = std::min(chunk, cmax);
+  }
   const int mpad_max = (chunk + NB - 1) / NB * NB;
-  double *dXn = nullptr, *dRon = nullptr, *dC = nullptr, *dout = nullptr, *dcoef = nullptr;
+  double *dXn = nullptr, *dRon = nullptr, *dC = nullptr, *dout = nullptr, *dcoef = nullptr, *dEc = nullptr,
+         *dCov = nullptr, *dDR = nullptr, *dDW = nullptr, *dDm = nullptr, *dDv = nullptr;
   auto cleanup = [&] {
-    if (dXn) hipFree(dXn);
-    if (dRon) hipFree(dRon);
-    if (dC) hipFree(dC);
-    if (dout) hipFree(dout);
-    if (dcoef) hipFree(dcoef);
+    for (double* q : {dXn, dRon, dC, dout, dcoef, dEc, dCov, dDR, dDW, dDm, dDv})
+      if (q) hipFree(q);
   };
   try {
     hip_check(hipMalloc(&dXn, sizeof(double) * (size_t)chunk * d), "hipMalloc");
@@ -691,13 +702,27 @@ void Kriging::predict(const double* Xn, int m, bool want_stdev, double* mean, do
     hip_check(hipMalloc(&dC, sizeof(double) * (size_t)npad_ * mpad_max), "hipMalloc");
     hip_check(hipMalloc(&dout, sizeof(double) * 2 * (size_t)chunk), "hipMalloc");
     hip_check(hipMalloc(&dcoef, sizeof(double) * (size_t)(p + p * p + 1)), "hipMalloc");
+    if ((want_cov || want_deriv) && p > 0)
+      hip_check(hipMalloc(&dEc, sizeof(double) * (size_t)chunk * p), "hipMalloc");
+    if (want_cov) hip_check(hipMalloc(&dCov, sizeof(double) * (size_t)m * m), "hipMalloc");
+    if (want_deriv) {
+      hip_check(hipMalloc(&dDR, sizeof(double) * (size_t)npad_ * mpad_max * d), "hipMalloc");
+      hip_check(hipMalloc(&dDW, sizeof(double) * (size_t)npad_ * mpad_max * d), "hipMalloc");
+      hip_check(hipMalloc(&dDm, sizeof(double) * (size_t)chunk * d), "hipMalloc");
+      hip_check(hipMalloc(&dDv, sizeof(double) * (size_t)chunk * d), "hipMalloc");
+    }
     std::vector<double> coef(p + p * p);
     for (int a = 0; a < p; ++a) coef[a] = beta_[a];
     for (int q = 0; q < p * p; ++q) coef[p + q] = Rstar_[q];
     if (p > 0) hip_check(hipMemcpy(dcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice), "upload");
     const bool nug = noise_ == NoiseModel::Nugget;
     const double var_scale = nug ? sigma2_ / alpha_ : sigma2_;
-    std::vector<double> xn((size_t)chunk * d), host_out(2 * (size_t)chunk);
+    const double vs = var_scale * scaleY_ * scaleY_;
+    std::vector<double> xn((size_t)chunk * d), host_out(2 * (size_t)chunk), hdm, hdv;
+    if (want_deriv) {
+      hdm.resize((size_t)chunk * d);
+      hdv.resize((size_t)chunk * d);
+    }
     for (int s0 = 0; s0 < m; s0 += chunk) {
       const int mc = std::min(chunk, m - s0);
       for (int k = 0; k < d; ++k)
@@ -712,6 +737,13 @@ void Kriging::predict(const double* Xn, int m, bool want_stdev, double* mean, do
       a.Ron = dRon;
       a.C = dC;
       a.out = dout;
+      a.ecirc = dEc;
+      a.cov = dCov;
+      a.nn_factor = nug ? alpha_ : 1.0;
+      a.DR = dDR;
+      a.DW = dDW;
+      a.dmean = dDm;
+      a.dvar = dDv;
       for (int k = 0; k < d; ++k) a.theta[k] = theta_[k];
       a.factor = nug ? alpha_ : 1.0;
       a.n = n_;
@@ -722,16 +754,45 @@ void Kriging::predict(const double* Xn, int m, bool want_stdev, double* mean, do
       a.mpad = (mc + NB - 1) / NB * NB;
       a.kernel = kernel_id(kernel_);
       a.linear_trend = regmodel_ == RegModel::Linear;
+      a.want_var = want_stdev ? 1 : 0;
       hip_check(launch_predict(a, rt_->stream()), "predict");
+      if (want_cov) hip_check(launch_predict_cov(a, rt_->stream()), "predict cov");
+      if (want_deriv) hip_check(launch_predict_deriv(a, rt_->stream()), "predict deriv");
       hip_check(hipMemcpyAsync(host_out.data(), dout, sizeof(double) * 2 * mc, hipMemcpyDeviceToHost, rt_->stream()),
                 "download");
+      if (want_deriv) {
+        hip_check(hipMemcpyAsync(hdm.data(), dDm, sizeof(double) * (size_t)mc * d, hipMemcpyDeviceToHost,
+                                 rt_->stream()),
+                  "download");
+        if (want_stdev)
+          hip_check(hipMemcpyAsync(hdv.data(), dDv, sizeof(double) * (size_t)mc * d, hipMemcpyDeviceToHost,
+                                   rt_->stream()),
+                    "download");
+      }
+      if (want_cov && cov != nullptr)
+        hip_check(hipMemcpyAsync(cov, dCov, sizeof(double) * (size_t)m * m, hipMemcpyDeviceToHost, rt_->stream()),
+                  "download");
       hip_check(hipStreamSynchronize(rt_->stream()), "sync");
       for (int j = 0; j < mc; ++j) {
         mean[s0 + j] = centerY_ + scaleY_ * host_out[j];
-        if (want_stdev && stdev != nullptr)
-          stdev[s0 + j] = std::sqrt(host_out[mc + j] * var_scale * scaleY_ * scaleY_);
+        if (want_stdev && stdev != nullptr) stdev[s0 + j] = std::sqrt(host_out[mc + j] * vs);
+      }
+      if (want_deriv) {
+        // chain rule of the normalisation (KrigingImpl.cpp:289-293), then
+        // d stdev = d var / (2 stdev)
+        for (int k = 0; k < d; ++k)
+          for (int j = 0; j < mc; ++j) {
+            if (mean_deriv != nullptr) mean_deriv[s0 + j + (size_t)k * m] = hdm[j + (size_t)k * mc] * scaleY_ / scaleX_[k];
+            if (stdev_deriv != nullptr) {
+              const double sd = std::sqrt(host_out[mc + j] * vs);
+              const double dv = want_stdev ? hdv[j + (size_t)k * mc] * vs / scaleX_[k] : 0.0;
+              stdev_deriv[s0 + j + (size_t)k * m] = dv / (2.0 * (want_stdev ? sd : 0.0));
+            }
+          }
       }
     }
+    if (want_cov && cov != nullptr)
+      for (size_t q = 0; q < (size_t)m * m; ++q) cov[q] *= vs;
   } catch (...) {
     cleanup();
     throw;

@@ -91,8 +91,12 @@ class Kriging {
   void logLikelihoodFunBatch(int nb, const double* gamma, int gamma_n, bool want_grad, double* ll, double* grad);
   double logLikelihood();
 
-  // Xn: m x d column-major; mean[m], stdev[m] (stdev may be null)
-  void predict(const double* Xn, int m, bool want_stdev, double* mean, double* stdev);
+  // Xn: m x d column-major; mean[m], stdev[m] (stdev may be null).
+  // cov: m x m; mean_deriv, stdev_deriv: m x d (column-major), when asked
+  // (KrigingImpl::predict_impl, KrigingImpl.cpp:145-300)
+  void predict(const double* Xn, int m, bool want_stdev, double* mean, double* stdev, bool want_cov = false,
+               double* cov = nullptr, bool want_deriv = false, double* mean_deriv = nullptr,
+               double* stdev_deriv = nullptr);
 
   // getters (normalised space, as the reference's members)
   const std::string& kernel() const { return kernel_; }

@@ -140,12 +140,9 @@ int lk_kriging_predict(void* ptr, const double* X_n, int m, int d, int return_st
     if (d != k->d())
       throw std::runtime_error("Predict locations have wrong dimension: " + std::to_string(d) + " instead of " +
                                std::to_string(k->d()));
-    if (return_cov || return_deriv)
-      throw std::invalid_argument("return_cov / return_deriv are not available on this path");
-    (void)cov_out;
-    (void)mean_deriv_out;
-    (void)stdev_deriv_out;
-    k->predict(X_n, m, return_stdev != 0, mean_out, return_stdev ? stdev_out : nullptr);
+    k->predict(X_n, m, return_stdev != 0, mean_out, return_stdev ? stdev_out : nullptr, return_cov != 0,
+               return_cov ? cov_out : nullptr, return_deriv != 0, return_deriv ? mean_deriv_out : nullptr,
+               return_deriv ? stdev_deriv_out : nullptr);
   });
 }
 

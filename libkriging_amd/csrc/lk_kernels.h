@@ -137,12 +137,26 @@ struct PredictArgs {
   double* Ron;         // npad x mpad workspace: R(x_obs, x_new)
   double* C;           // npad x mpad workspace: L^-1 R_on
   double* out;         // 2 m: mean (normalised), variance factor
+  double* ecirc;       // m x p (ld = m): E circ rows, when cov or derivatives are wanted (else null)
+  // covariance (KrigingImpl.cpp:245-253): m x m, ld = m, before var_scale
+  double* cov;
+  double nn_factor;    // R_nn_factor (alpha for the nugget model); diagonal 1
+  // derivatives (KrigingImpl.cpp:255-291): dR_on / dx_new and L^-1 of it,
+  // npad x (d mpad) with column k mpad + j for point j and coordinate k;
+  // dmean, dvar: m x d (ld = m), normalised units
+  double* DR;
+  double* DW;
+  double* dmean;
+  double* dvar;
   double theta[DMAX];
   double factor;       // R_on_factor (alpha for the nugget model)
   int n, d, npad, p, m, mpad;
   int kernel, linear_trend;
+  int want_var;        // derivatives of the variance too
 };
 hipError_t launch_predict(const PredictArgs& a, hipStream_t stream);
+hipError_t launch_predict_cov(const PredictArgs& a, hipStream_t stream);    // after launch_predict (same chunk)
+hipError_t launch_predict_deriv(const PredictArgs& a, hipStream_t stream);  // after launch_predict (same chunk)
 // plain device copy of `bytes` (a multiple of 16)
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
 

@@ -1149,6 +1149,8 @@ __global__ __launch_bounds__(TPB) void k_predict_reduce(PredictArgs a) {
     e[q] = t / Rs[q + q * p];
     ee = fma(e[q], e[q], ee);
   }
+  if (a.ecirc != nullptr)
+    for (int q = 0; q < p; ++q) a.ecirc[(size_t)q * a.m + j] = e[q];
   double v = 1.0 - ss + ee;
   if (isnan(v) || v < 0.0) v = 0.0;
   a.out[j] = mean;
@@ -1164,6 +1166,195 @@ __global__ __launch_bounds__(TPB) void k_copy(const double2* __restrict__ src, d
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream) {
   const size_t n = bytes / 16;
   hipLaunchKernelGGL(k_copy, dim3(4096), dim3(TPB), 0, stream, (const double2*)src, (double2*)dst, n);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// (10) predictive covariance: Sigma = R_nn - C' C + Ecirc Ecirc' (before
+// var_scale), C = L^-1 R_on.  R_nn as LinearAlgebra::covMat_sym_X
+// (LinearAlgebra.cpp:531-596): factor * k(xn_i - xn_j) off the diagonal, 1
+// on it.  grid (mpad / 64, mpad / 64), full matrix
+// ---------------------------------------------------------------------------
+template <int DT>
+__global__ __launch_bounds__(TPB) void k_predict_cov(PredictArgs a) {
+  const int j0 = blockIdx.x * NB, i0 = blockIdx.y * NB;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  const int m = a.m, d = a.d;
+  __shared__ double sbuf[4 * KC * LDSS];
+  d4 acc[2][2];
+  acc_zero(acc);
+  tile_mainloop<true, true, true>(a.C, a.npad, i0, a.C, a.npad, j0, 0, a.npad, sbuf, acc, tid, lane, wr, wc);
+  double* sXi = sbuf;
+  double* sXj = sbuf + DT * NB;
+  for (int q = tid; q < DT * NB; q += TPB) {
+    const int k = q / NB, r = q % NB;
+    sXi[q] = (k < d && i0 + r < m) ? a.Xn[(size_t)k * m + i0 + r] : 0.0;
+    sXj[q] = (k < d && j0 + r < m) ? a.Xn[(size_t)k * m + j0 + r] : 0.0;
+  }
+  __syncthreads();
+  double it[DT];
+#pragma unroll
+  for (int k = 0; k < DT; ++k) it[k] = (k < d) ? 1.0 / a.theta[k] : 1.0;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = acc_row(lane, wr, bi), c = acc_col(lane, wc, bj, e);
+        const int gi = i0 + r, gj = j0 + c;
+        if (gi >= m || gj >= m) continue;
+        double rnn = 1.0;
+        if (gi != gj) {
+          double dx[DT];
+#pragma unroll
+          for (int k = 0; k < DT; ++k) dx[k] = sXi[k * NB + r] - sXj[k * NB + c];
+          rnn = kern_rho<DT>(a.kernel, dx, it) * a.nn_factor;
+        }
+        double ec = 0.0;
+        for (int q = 0; q < a.p; ++q) ec = fma(a.ecirc[(size_t)q * m + gi], a.ecirc[(size_t)q * m + gj], ec);
+        a.cov[(size_t)gj * m + gi] = acc[bi][bj][e] + rnn + ec;
+      }
+}
+
+// d ln k / d dx_k (Covariance.cpp:52-63, 90-101, 132-146, 181-198), dx = x_new - x_obs
+__device__ inline double dlncov_dx(int kern, double dx, double th) {
+  const double sg = dx > 0.0 ? 1.0 : (dx < 0.0 ? -1.0 : 0.0);
+  if (kern == K_GAUSS) return -dx / (th * th);
+  if (kern == K_EXP) return -sg / th;
+  if (kern == K_MATERN32) {
+    const double u = SQRT3 * fabs(dx / th);
+    return -SQRT3 * sg * u / (1.0 + u) / th;
+  }
+  const double u = SQRT5 * fabs(dx / th);
+  const double aa = 1.0 + u, bb = u / 3.0;
+  return -SQRT5 * sg * (aa * bb) / (aa + u * bb) / th;
+}
+
+// DR(i, k mpad + j) = R_on(i, j) d ln k / d dx_k (xn_j - x_i); zero padding.
+// R_on as k_cross_cov (1 at coincident points, where the derivative is 0).
+// grid (mpad / 64, npad / 64)
+template <int DT>
+__global__ __launch_bounds__(TPB) void k_predict_dron(PredictArgs a) {
+  const int j0 = blockIdx.x * NB, i0 = blockIdx.y * NB, tid = threadIdx.x;
+  const int n = a.n, m = a.m, d = a.d;
+  __shared__ double sXo[DMAX * NB];
+  __shared__ double sXn[DMAX * NB];
+  for (int q = tid; q < d * NB; q += TPB) {
+    const int k = q / NB, r = q % NB;
+    sXo[q] = (i0 + r < n) ? a.X[(size_t)k * n + i0 + r] : 0.0;
+    sXn[q] = (j0 + r < m) ? a.Xn[(size_t)k * m + j0 + r] : 0.0;
+  }
+  double it[DT];
+#pragma unroll
+  for (int k = 0; k < DT; ++k) it[k] = (k < d) ? 1.0 / a.theta[k] : 1.0;
+  __syncthreads();
+  const int ii = tid & 63, gi = i0 + ii;
+  double dx[DT];
+  for (int r = 0; r < NB / 4; ++r) {
+    const int jj = (tid >> 6) + 4 * r, gj = j0 + jj;
+    double v = 0.0;
+    if (gi < n && gj < m) {
+      bool same = true;
+#pragma unroll
+      for (int k = 0; k < DT; ++k) {
+        dx[k] = (k < d) ? sXo[k * NB + ii] - sXn[k * NB + jj] : 0.0;
+        same = same && fabs(dx[k]) <= 2.220446049250313e-16;
+      }
+      v = same ? 1.0 : kern_rho<DT>(a.kernel, dx, it) * a.factor;
+    }
+    for (int k = 0; k < d; ++k) {
+      const double g = (gi < n && gj < m) ? dlncov_dx(a.kernel, sXn[k * NB + jj] - sXo[k * NB + ii], a.theta[k]) : 0.0;
+      a.DR[((size_t)k * a.mpad + gj) * a.npad + gi] = v * g;
+    }
+  }
+}
+
+// one wave per point j: per coordinate k, with W = L^-1 DR,
+//   dmean_k = (dF_j beta)_k + W(:, k j)' z
+//   dvar_k  = -2 C(:, j)' W(:, k j) + 2 Ecirc_j DEcirc(:, k),
+//   DEcirc(:, k) = solve_lower(R*', dF_j(k, :)' - M' W(:, k j))
+// (KrigingImpl.cpp:276-288).  grid ceil(m / 4)
+__global__ __launch_bounds__(TPB) void k_predict_deriv(PredictArgs a) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int j = blockIdx.x * 4 + w;
+  if (j >= a.m) return;
+  const int n = a.n, ld = a.npad, p = a.p, m = a.m;
+  const double* z = a.MZ + (size_t)p * ld;
+  const double* cj = a.C + (size_t)j * ld;
+  const double* beta = a.coef;
+  const double* Rs = a.coef + p;
+  for (int k = 0; k < a.d; ++k) {
+    const double* wk = a.DW + ((size_t)k * a.mpad + j) * ld;
+    double sz = 0.0, sc = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      const double wv = wk[i];
+      sz = fma(wv, z[i], sz);
+      sc = fma(wv, cj[i], sc);
+    }
+    sz = wave_sum(sz);
+    sc = wave_sum(sc);
+    double dm = sz;
+    if (a.linear_trend) dm += beta[k + 1];  // dF(k, q) = 1 for q = k + 1
+    double dv = 0.0;
+    if (a.want_var) {
+      double e[PMAX];
+      for (int q = 0; q < p; ++q) {
+        double s = 0.0;
+        for (int i = lane; i < n; i += 64) s = fma(wk[i], a.MZ[(size_t)q * ld + i], s);
+        s = wave_sum(s);
+        const double df = (a.linear_trend && q == k + 1) ? 1.0 : 0.0;
+        e[q] = df - s;
+      }
+      double ed = 0.0;
+      for (int q = 0; q < p; ++q) {  // R*' x = e (forward substitution)
+        double t = e[q];
+        for (int r = 0; r < q; ++r) t -= Rs[r + q * p] * e[r];
+        e[q] = t / Rs[q + q * p];
+        ed = fma(a.ecirc[(size_t)q * m + j], e[q], ed);
+      }
+      dv = -2.0 * sc + 2.0 * ed;
+    }
+    if (lane == 0) {
+      a.dmean[(size_t)k * m + j] = dm;
+      if (a.want_var) a.dvar[(size_t)k * m + j] = dv;
+    }
+  }
+}
+
+hipError_t launch_predict_cov(const PredictArgs& a, hipStream_t stream) {
+  const dim3 grid(a.mpad / NB, a.mpad / NB);
+  if (a.d <= 4)
+    hipLaunchKernelGGL(k_predict_cov<4>, grid, dim3(TPB), 0, stream, a);
+  else if (a.d <= 8)
+    hipLaunchKernelGGL(k_predict_cov<8>, grid, dim3(TPB), 0, stream, a);
+  else if (a.d <= 16)
+    hipLaunchKernelGGL(k_predict_cov<16>, grid, dim3(TPB), 0, stream, a);
+  else
+    hipLaunchKernelGGL(k_predict_cov<32>, grid, dim3(TPB), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_predict_deriv(const PredictArgs& a, hipStream_t stream) {
+  const dim3 grid(a.mpad / NB, a.npad / NB);
+  if (a.d <= 4)
+    hipLaunchKernelGGL(k_predict_dron<4>, grid, dim3(TPB), 0, stream, a);
+  else if (a.d <= 8)
+    hipLaunchKernelGGL(k_predict_dron<8>, grid, dim3(TPB), 0, stream, a);
+  else if (a.d <= 16)
+    hipLaunchKernelGGL(k_predict_dron<16>, grid, dim3(TPB), 0, stream, a);
+  else
+    hipLaunchKernelGGL(k_predict_dron<32>, grid, dim3(TPB), 0, stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // W = L^-1 DR: the lower TRMM over d mpad columns
+  PredictArgs t = a;
+  t.Ron = a.DR;
+  t.C = a.DW;
+  t.mpad = a.mpad * a.d;
+  hipLaunchKernelGGL(k_trmm_lower, dim3(t.mpad / NB, a.npad / NB), dim3(TPB), 0, stream, t);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_predict_deriv, dim3((a.m + 3) / 4), dim3(TPB), 0, stream, a);
   return hipGetLastError();
 }
 

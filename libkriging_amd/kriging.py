@@ -160,15 +160,23 @@ class Kriging:
             P["be"].size if P["be"] is not None else 0, P["be_e"], P["nu"], P["nu_e"]))
 
     def predict(self, X, return_stdev=True, return_cov=False, return_deriv=False):
+        """(mean, stdev, cov, mean_deriv, stdev_deriv) as Kriging::predict
+        (Kriging.cpp:2241-2285); unrequested outputs are empty"""
         X = _mat(X)
         m, d = X.shape
         mean = np.zeros(m)
         sd = np.zeros(m) if return_stdev else None
+        cov = np.zeros((m, m), order="F") if return_cov else None
+        dm = np.zeros((m, d), order="F") if return_deriv else None
+        ds = np.zeros((m, d), order="F") if return_deriv else None
+
+        def p_(a):
+            return N.ptr(a) if a is not None else None
         N.check(N.lib().lk_kriging_predict(self._h, N.ptr(X), m, d, int(return_stdev), int(return_cov),
-                                           int(return_deriv), N.ptr(mean), N.ptr(sd) if sd is not None else None,
-                                           None, None, None))
+                                           int(return_deriv), N.ptr(mean), p_(sd), p_(cov), p_(dm), p_(ds)))
         empty = np.zeros((0, 0))
-        return mean, (sd if sd is not None else np.zeros(0)), empty, empty, empty
+        return (mean, (sd if sd is not None else np.zeros(0)), cov if cov is not None else empty,
+                dm if dm is not None else empty, ds if ds is not None else empty)
 
     def logLikelihoodFun(self, theta, return_grad=False, want_hess=False):
         th = _col(theta)
