@@ -141,6 +141,13 @@ int lkgpu_gradient_path(lkgpu_context* ctx);
  * FETCH_SIZE / WRITE_SIZE counters behind the roofline traffic figures. */
 int lkgpu_device_copy(lkgpu_context* ctx, size_t bytes, int reps, double* ms_out);
 
+/* FP64 MFMA issue-rate probe: `blocks` workgroups of 4 waves, each wave
+ * running `iters` rounds of `chains` (1, 2, 4 or 8) independent
+ * 16x16x4 FP64 MFMA accumulations from registers; *ms_out = kernel time,
+ * *tflops_out = achieved FP64 TFLOP/s.  No counterpart in the reference: the
+ * measured ceiling beside the published 78.6 TFLOP/s peak. */
+int lkgpu_mfma_probe(lkgpu_context* ctx, int blocks, int iters, int chains, double* ms_out, double* tflops_out);
+
 /* ------------------------------------------------------------------------ */
 /* Layer 2: the Kriging object (libkriging_c.h names and conventions)       */
 /* Supported here: kernels gauss/exp/matern3_2/matern5_2, noise "none" or   */

@@ -24,6 +24,8 @@ SIGNATURES = {
     "lkgpu_set_gradient_path": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lkgpu_gradient_path": (ctypes.c_int, [_vp]),
     "lkgpu_device_copy": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "lkgpu_mfma_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double)]),
     "lk_get_last_error": (ctypes.c_char_p, []),
     "lkgpu_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "lkgpu_context_destroy": (None, [_vp]),
@@ -177,6 +179,13 @@ class Context:
         ms = ctypes.c_double()
         check(lib().lkgpu_device_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
         return ms.value
+
+    def mfma_probe(self, blocks, iters, chains=4):
+        """(ms, TFLOP/s) of the FP64 MFMA issue-rate probe"""
+        ms, tf = ctypes.c_double(), ctypes.c_double()
+        check(lib().lkgpu_mfma_probe(self.h, int(blocks), int(iters), int(chains), ctypes.byref(ms),
+                                     ctypes.byref(tf)))
+        return ms.value, tf.value
 
     def take_stage_time(self):
         ms = ctypes.c_double()

@@ -13,6 +13,9 @@ namespace lk {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int KC = 16;          // K-chunk staged in LDS per step
+#ifndef LK_PREFETCH_DEPTH
+#define LK_PREFETCH_DEPTH 3
+#endif
 // LDS row stride (doubles) of a KC x NB staging tile.  66: the transposed
 // staging (16 threads writing 16 different k rows of one column pair) then
 // falls on 16 distinct even banks (2 * 2 * 66 = 264 = 8 mod 32 words), no
@@ -205,20 +208,36 @@ __device__ inline void tile_mainloop(const double* __restrict__ A, int lda, int 
   if (kbeg >= kend) return;
   double* sA[2] = {sbuf, sbuf + 2 * KC * LDSS};
   double* sB[2] = {sbuf + KC * LDSS, sbuf + 3 * KC * LDSS};
-  ChunkRegs ra, rb;
-  chunk_load<TA>(ra, A, lda, ai0, kbeg, tid);
-  chunk_load<TB>(rb, B, ldb, bj0, kbeg, tid);
-  int buf = 0;
-  for (int k0 = kbeg; k0 < kend; k0 += KC) {
-    chunk_store<TA>(ra, sA[buf], tid);
-    chunk_store<TB>(rb, sB[buf], tid);
-    __syncthreads();
-    if (k0 + KC < kend) {  // next chunk's loads overlap this chunk's MFMAs
-      chunk_load<TA>(ra, A, lda, ai0, k0 + KC, tid);
-      chunk_load<TB>(rb, B, ldb, bj0, k0 + KC, tid);
+  // Chunk t is staged from register set t % D into LDS buffer t % 2 and
+  // chunk t + D is then loaded into the same registers, so D chunks are in
+  // flight and each load's latency is covered by the MFMAs of D chunks.
+  // Unrolled over lcm(D, 2) chunks so that register sets stay static.
+  constexpr int D = LK_PREFETCH_DEPTH;
+  static_assert(D >= 1 && D <= 4, "prefetch depth");
+  ChunkRegs ra[D], rb[D];
+  const int nch = (kend - kbeg) / KC;
+#pragma unroll
+  for (int q = 0; q < D; ++q)
+    if (q < nch) {
+      chunk_load<TA>(ra[q], A, lda, ai0, kbeg + q * KC, tid);
+      chunk_load<TB>(rb[q], B, ldb, bj0, kbeg + q * KC, tid);
     }
-    mma_chunk<NEG>(sA[buf], sB[buf], acc, lane, wr, wc);
-    buf ^= 1;
+  constexpr int U = (D % 2 == 0) ? D : 2 * D;
+  for (int t0 = 0; t0 < nch; t0 += U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = t0 + u;
+      if (t >= nch) break;
+      const int lb = u & 1;
+      chunk_store<TA>(ra[u % D], sA[lb], tid);
+      chunk_store<TB>(rb[u % D], sB[lb], tid);
+      __syncthreads();
+      if (t + D < nch) {
+        chunk_load<TA>(ra[u % D], A, lda, ai0, kbeg + (t + D) * KC, tid);
+        chunk_load<TB>(rb[u % D], B, ldb, bj0, kbeg + (t + D) * KC, tid);
+      }
+      mma_chunk<NEG>(sA[lb], sB[lb], acc, lane, wr, wc);
+    }
   }
   // the caller may reuse sbuf: wait until every wave has finished reading it
   __syncthreads();

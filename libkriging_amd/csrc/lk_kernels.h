@@ -159,6 +159,8 @@ hipError_t launch_predict_cov(const PredictArgs& a, hipStream_t stream);    // a
 hipError_t launch_predict_deriv(const PredictArgs& a, hipStream_t stream);  // after launch_predict (same chunk)
 // plain device copy of `bytes` (a multiple of 16)
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
+// chains in {1, 2, 4, 8}; out: blocks * 256 doubles
+hipError_t launch_mfma_probe(int blocks, int iters, int chains, double* out, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // NestedKriging "NK" aggregation (NestedKriging::predict_nk,
