@@ -11,6 +11,7 @@
 namespace lk {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));  // loads as one 16-byte vector
 
 constexpr int KC = 16;          // K-chunk staged in LDS per step
 #ifndef LK_PREFETCH_DEPTH
@@ -26,6 +27,16 @@ constexpr int TPB = 256;        // threads of the tile kernels (4 waves)
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
+// Loads through a global-address-space pointer.  Matrix pointers reach the
+// kernels inside slot descriptors, so the compiler cannot prove that they
+// point to global memory and emits FLAT loads, which count against the LDS
+// counter too: every LDS wait of the MFMA loop would then also wait for the
+// prefetch loads in flight.  GLOBAL loads count against vmcnt only.
+template <class T>
+__device__ inline T gload(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 __device__ inline double readlane_d(double v, int lane) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
@@ -116,7 +127,7 @@ __device__ inline void mma_chunk(const double* __restrict__ sA, const double* __
 constexpr int CHUNK_V2 = KC * NB / (2 * TPB);  // double2 per thread per KC x NB chunk
 static_assert(KC % 8 == 0 && CHUNK_V2 >= 1, "chunk layout");
 struct ChunkRegs {
-  double2 v[CHUNK_V2];
+  d2 v[CHUNK_V2];
 };
 
 // TRANS = false: a thread copies pairs of rows (ii, ii + 1) of k rows
@@ -129,13 +140,13 @@ __device__ inline void chunk_load(ChunkRegs& r, const double* __restrict__ M, in
     const int ii = 2 * (tid & 31);
 #pragma unroll
     for (int q = 0; q < CHUNK_V2; ++q)
-      r.v[q] = *reinterpret_cast<const double2*>(M + (size_t)(k0 + (tid >> 5) + 8 * q) * ld + i0 + ii);
+      r.v[q] = gload(reinterpret_cast<const d2*>(M + (size_t)(k0 + (tid >> 5) + 8 * q) * ld + i0 + ii));
   } else {
     constexpr int H = KC / 2;
     const int kk = 2 * (tid % H);
 #pragma unroll
     for (int q = 0; q < CHUNK_V2; ++q)
-      r.v[q] = *reinterpret_cast<const double2*>(M + (size_t)(i0 + tid / H + (TPB / H) * q) * ld + k0 + kk);
+      r.v[q] = gload(reinterpret_cast<const d2*>(M + (size_t)(i0 + tid / H + (TPB / H) * q) * ld + k0 + kk));
   }
 }
 
@@ -188,7 +199,7 @@ __device__ inline void acc_load(d4 (&acc)[2][2], const double* __restrict__ C, i
     for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
       for (int e = 0; e < 4; ++e)
-        acc[bi][bj][e] = C[(size_t)acc_col(lane, wc, bj, e) * ld + acc_row(lane, wr, bi)];
+        acc[bi][bj][e] = gload(C + (size_t)acc_col(lane, wc, bj, e) * ld + acc_row(lane, wr, bi));
 }
 
 __device__ inline void acc_store(const d4 (&acc)[2][2], double* __restrict__ C, int ld, int lane, int wr, int wc) {

@@ -429,29 +429,53 @@ __global__ __launch_bounds__(TPB) void k_trtri(const Slot* __restrict__ slots, c
 // (4) triangular products with the explicit inverse.
 // V columns: [0, p) F*, p y*, p+1 r, p+2 E*, p+3 x, p+4.. p+6 estimator scratch
 // ---------------------------------------------------------------------------
+// The matrix-vector kernels stream the L^-1 (and L) triangles once; they
+// run 8 waves per workgroup and issue MV_U independent loads per lane
+// before the first dependent FMA, so a workgroup keeps ~32 KB in flight
+// instead of one 512-byte line per wave.
+constexpr int MV_TPB = 512;
+constexpr int MV_W = MV_TPB / 64;
+constexpr int MV_U = 8;
+
 // out[:, c] = Linv * in[:, c] for rows of block rb (grid (nblk_max, nslots))
 // in columns come from `src` (n rows, ld = n, zero beyond n) ; out to V.
 // Column c of the input is src[c * lds + k] for c < ncols_src and src2[k]
 // for c == ncols_src (when src2 is given), so [F | y] go through in one pass.
+// Wave w sums columns k in batches [w U + j W U, w U + j W U + U); the
+// partial sums of the waves are added in a fixed order (deterministic).
 __device__ inline void lower_matvec_block(const Slot& S, int rb, const double* __restrict__ src, int lds, int nsrc,
                                           int ncols, double* __restrict__ dst, int ldd, double* __restrict__ part_sq,
                                           const double* __restrict__ src2 = nullptr) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int i0 = rb * NB, ld = S.npad, n = S.n;
   const int kend = min(i0 + NB, n);  // columns k <= row, and k < n (src is zero beyond n)
-  __shared__ double red[4][4][NB];
+  __shared__ double red[MV_W][4][NB];
+  const double* __restrict__ Wr = S.W + i0 + lane;
   double ssq = 0.0;
   for (int c0 = 0; c0 < ncols; c0 += 4) {
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
     const int nc = min(4, ncols - c0);
-#pragma unroll 8
-    for (int k = w; k < kend; k += 4) {
-      const double lv = S.W[(size_t)k * ld + i0 + lane];
+    // columns past nc re-read column c0 (same cache lines) and are dropped
+    const double* colp[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (c < nc) {
-          const double* col = (src2 != nullptr && c0 + c == ncols - 1) ? src2 : src + (size_t)(c0 + c) * lds;
-          acc[c] = fma(lv, col[k], acc[c]);
+    for (int c = 0; c < 4; ++c) {
+      const int cs = c0 + (c < nc ? c : 0);
+      colp[c] = (src2 != nullptr && cs == ncols - 1) ? src2 : src + (size_t)cs * lds;
+    }
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int kb = w * MV_U; kb < kend; kb += MV_W * MV_U) {
+      double lv[MV_U], sv[MV_U][4];
+#pragma unroll
+      for (int u = 0; u < MV_U; ++u) {
+        const int k = min(kb + u, kend - 1);
+        lv[u] = gload(Wr + (size_t)k * ld);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sv[u][c] = gload(colp[c] + k);
+      }
+#pragma unroll
+      for (int u = 0; u < MV_U; ++u)
+        if (kb + u < kend) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) acc[c] = fma(lv[u], sv[u][c], acc[c]);
         }
     }
 #pragma unroll
@@ -463,7 +487,9 @@ __device__ inline void lower_matvec_block(const Slot& S, int rb, const double* _
         if (c >= nc) continue;
         // rows above the diagonal read W's upper part of the diagonal block,
         // which is exactly zero; rows >= n are identity rows times zero input.
-        const double v = red[0][c][lane] + red[1][c][lane] + red[2][c][lane] + red[3][c][lane];
+        double v = red[0][c][lane];
+#pragma unroll
+        for (int q = 1; q < MV_W; ++q) v += red[q][c][lane];
         dst[(size_t)(c0 + c) * ldd + i0 + lane] = v;
         ssq = fma(v, v, ssq);
       }
@@ -478,7 +504,7 @@ __device__ inline void lower_matvec_block(const Slot& S, int rb, const double* _
 }
 
 // [F* | y*] = Linv [F | y]
-__global__ __launch_bounds__(TPB) void k_solve_fy(const Slot* __restrict__ slots,
+__global__ __launch_bounds__(MV_TPB) void k_solve_fy(const Slot* __restrict__ slots,
                                                   const SlotParams* __restrict__ params,
                                                   SlotOut* __restrict__ out) {
   const TileSlot ts = tile_slot();
@@ -565,7 +591,7 @@ __global__ __launch_bounds__(TPB) void k_gls(const Slot* __restrict__ slots, con
 }
 
 // E* = Linv r, SSE partials
-__global__ __launch_bounds__(TPB) void k_solve_e(const Slot* __restrict__ slots,
+__global__ __launch_bounds__(MV_TPB) void k_solve_e(const Slot* __restrict__ slots,
                                                  const SlotParams* __restrict__ params,
                                                  SlotOut* __restrict__ out) {
   const TileSlot ts = tile_slot();
@@ -578,7 +604,7 @@ __global__ __launch_bounds__(TPB) void k_solve_e(const Slot* __restrict__ slots,
 }
 
 // x = Linv^T E*, and the 1-norms (max column abs sums) of L and Linv
-__global__ __launch_bounds__(TPB) void k_solve_x(const Slot* __restrict__ slots,
+__global__ __launch_bounds__(MV_TPB) void k_solve_x(const Slot* __restrict__ slots,
                                                  const SlotParams* __restrict__ params,
                                                  SlotOut* __restrict__ out) {
   const TileSlot ts = tile_slot();
@@ -589,15 +615,29 @@ __global__ __launch_bounds__(TPB) void k_solve_x(const Slot* __restrict__ slots,
   const double* e = S.V + (size_t)(S.p + 2) * ld;
   double* x = S.V + (size_t)(S.p + 3) * ld;
   double mW = 0.0, mL = 0.0;
-  for (int cc = w; cc < NB; cc += 4) {
+  constexpr int XU = 4;  // 3 XU independent loads per lane per batch
+  for (int cc = w; cc < NB; cc += MV_W) {
     const int c = ts.tile * NB + cc;
+    const double* __restrict__ Wc = S.W + (size_t)c * ld;
+    const double* __restrict__ Ac = S.A + (size_t)c * ld;
     double s = 0.0, aw = 0.0, al = 0.0;
-#pragma unroll 4
-    for (int i = c + lane; i < ld; i += 64) {
-      const double wv = S.W[(size_t)c * ld + i];
-      s = fma(wv, e[i], s);
-      aw += fabs(wv);
-      al += fabs(S.A[(size_t)c * ld + i]);
+    for (int b0 = c; b0 < ld; b0 += 64 * XU) {
+      const int ib = b0 + lane;
+      double wv[XU], ev[XU], av[XU];
+#pragma unroll
+      for (int u = 0; u < XU; ++u) {
+        const int i = min(ib + 64 * u, ld - 1);
+        wv[u] = gload(Wc + i);
+        ev[u] = gload(e + i);
+        av[u] = gload(Ac + i);
+      }
+#pragma unroll
+      for (int u = 0; u < XU; ++u)
+        if (ib + 64 * u < ld) {
+          s = fma(wv[u], ev[u], s);
+          aw += fabs(wv[u]);
+          al += fabs(av[u]);
+        }
     }
     s = wave_sum(s);
     aw = wave_sum(aw);
@@ -608,15 +648,20 @@ __global__ __launch_bounds__(TPB) void k_solve_x(const Slot* __restrict__ slots,
       mL = fmax(mL, al);
     }
   }
-  __shared__ double sm[2][4];
+  __shared__ double sm[2][MV_W];
   if (lane == 0) {
     sm[0][w] = mW;
     sm[1][w] = mL;
   }
   __syncthreads();
   if (tid == 0) {
-    S.red[red_colW_off(S.nblk) + ts.tile] = fmax(fmax(sm[0][0], sm[0][1]), fmax(sm[0][2], sm[0][3]));
-    S.red[red_colL_off(S.nblk) + ts.tile] = fmax(fmax(sm[1][0], sm[1][1]), fmax(sm[1][2], sm[1][3]));
+    double a = sm[0][0], l = sm[1][0];
+    for (int q = 1; q < MV_W; ++q) {
+      a = fmax(a, sm[0][q]);
+      l = fmax(l, sm[1][q]);
+    }
+    S.red[red_colW_off(S.nblk) + ts.tile] = a;
+    S.red[red_colL_off(S.nblk) + ts.tile] = l;
   }
 }
 
@@ -1457,7 +1502,7 @@ hipError_t launch_trtri_level(const Batch& B, int s, int step) {
   return hipGetLastError();
 }
 hipError_t launch_solve_fy(const Batch& B) {
-  hipLaunchKernelGGL(k_solve_fy, dim3(B.nblk_max, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  hipLaunchKernelGGL(k_solve_fy, dim3(B.nblk_max, B.nslots), dim3(MV_TPB), 0, B.stream, B.slots, B.params, B.out);
   return hipGetLastError();
 }
 hipError_t launch_gls(const Batch& B) {
@@ -1465,11 +1510,11 @@ hipError_t launch_gls(const Batch& B) {
   return hipGetLastError();
 }
 hipError_t launch_solve_e(const Batch& B) {
-  hipLaunchKernelGGL(k_solve_e, dim3(B.nblk_max, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  hipLaunchKernelGGL(k_solve_e, dim3(B.nblk_max, B.nslots), dim3(MV_TPB), 0, B.stream, B.slots, B.params, B.out);
   return hipGetLastError();
 }
 hipError_t launch_solve_x(const Batch& B) {
-  hipLaunchKernelGGL(k_solve_x, dim3(B.nblk_max, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  hipLaunchKernelGGL(k_solve_x, dim3(B.nblk_max, B.nslots), dim3(MV_TPB), 0, B.stream, B.slots, B.params, B.out);
   return hipGetLastError();
 }
 hipError_t launch_finalize_ll(const Batch& B, bool rinv_path) {
