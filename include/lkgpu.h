@@ -99,6 +99,11 @@ int lkgpu_ll_grad_batch(lkgpu_context* ctx, int nb, const int* problem_ids, cons
 #define LKGPU_READ_YSTAR 3 /* n       KModel.ystar */
 #define LKGPU_READ_ESTAR 4 /* n       KModel.Estar */
 #define LKGPU_READ_X 5     /* n       L^-T E* = R^-1 (y - F beta) */
+#define LKGPU_READ_RSTAR 6 /* p x p   KModel.Rstar (upper) */
+/* after a leave-one-out evaluation (lk_kriging_leave_one_out_fun / _vec):
+ * errorsLOO and sigma2LOO of Kriging::_leaveOneOut (Kriging.cpp:409-412) */
+#define LKGPU_READ_LOO_ERRORS 7 /* n */
+#define LKGPU_READ_LOO_SIGMA2 8 /* n */
 int lkgpu_read_item(lkgpu_context* ctx, int item, int what, double* out);
 
 /* Live timing of one pipeline stage: a HIP event pair is recorded on the
@@ -178,7 +183,9 @@ int lk_kriging_fit_parameters(void* ptr, const double* y, int n, const double* X
                               int is_sigma2_estim, const double* theta, int theta_n, int is_theta_estim,
                               const double* beta, int beta_n, int is_beta_estim, const double* nugget,
                               int is_nugget_estim);
-/* libkriging_c.h:60-71; return_cov / return_deriv are rejected */
+/* libkriging_c.h:60-71 (Kriging::predict, Kriging.cpp:2241-2285): mean,
+ * stdev, the m x m covariance (at most 32768 points per call) and the
+ * m x d derivatives of mean and stdev, column-major */
 int lk_kriging_predict(void* ptr, const double* X_n, int m, int d, int return_stdev, int return_cov, int return_deriv,
                        double* mean_out, double* stdev_out, double* cov_out, double* mean_deriv_out,
                        double* stdev_deriv_out);
@@ -192,6 +199,15 @@ int lk_kriging_log_likelihood_fun_batch(void* ptr, int nb, const double* theta, 
                                         double* ll_out, double* grad_out);
 /* libkriging_c.h:131 */
 double lk_kriging_log_likelihood(void* ptr);
+/* libkriging_c.h:118-123 (Kriging::leaveOneOutFun, Kriging.cpp:470-476);
+ * theta has d entries; noise model none only */
+int lk_kriging_leave_one_out_fun(void* ptr, const double* theta, int theta_n, int return_grad, double* loo_out,
+                                 double* grad_out);
+/* libkriging_c.h:133 (Kriging::leaveOneOut at the fitted theta) */
+double lk_kriging_leave_one_out(void* ptr);
+/* libkriging_c.h:136 (Kriging::leaveOneOutVec, Kriging.cpp:478-485): n
+ * leave-one-out predictions and standard errors */
+int lk_kriging_leave_one_out_vec(void* ptr, const double* theta, int theta_n, double* yhat_out, double* stderr_out);
 
 /* getters, libkriging_c.h:141-166 (normalised-space members) */
 const char* lk_kriging_kernel(void* ptr);

@@ -18,6 +18,7 @@ SOURCES = [
     "lk_kernels.hip",
     "lk_nested.hip",
     "lk_rinv.hip",
+    "lk_loo.hip",
     "lk_runtime.cpp",
     "lk_capi.cpp",
     "lbfgsb.cpp",

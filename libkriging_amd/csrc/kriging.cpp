@@ -248,6 +248,46 @@ void Kriging::logLikelihoodFunBatch(int nb, const double* gamma, int gamma_n, bo
   }
 }
 
+// Kriging::leaveOneOutFun (Kriging.cpp:470-476): theta has d entries
+double Kriging::leaveOneOutFun(const double* theta, int theta_n, bool want_grad, double* grad) {
+  if (rt_ == nullptr) throw std::runtime_error("model is not fitted (no data)");
+  if (noise_ != NoiseModel::None)
+    throw std::invalid_argument("LOO objective not supported for Nugget/Heterogeneous noise modes");
+  if (theta_n != d_) throw std::invalid_argument("theta must have length " + std::to_string(d_));
+  std::vector<EvalRequest> reqs(1);
+  reqs[0].problem = problem_;
+  reqs[0].gamma = theta;
+  reqs[0].gamma_n = theta_n;
+  reqs[0].flags = flags_for_eval();
+  std::vector<EvalResult> res;
+  rt_->evaluate(reqs, want_grad, res, nullptr, OBJ_LOO);
+  if (res[0].status != EVAL_OK)
+    throw std::runtime_error("[ERROR] Exceed max numerical nugget (" + std::to_string(res[0].inc) +
+                             " x 1e-10) added to force chol matrix");
+  if (want_grad)
+    for (int k = 0; k < theta_n; ++k) grad[k] = res[0].grad[k];
+  return res[0].ll;
+}
+
+// Kriging::leaveOneOutVec (Kriging.cpp:478-485): yhat = y - errors and
+// sqrt(sigma2LOO) * sqrt(sigma2), in the model's (normalised) units
+void Kriging::leaveOneOutVec(const double* theta, int theta_n, double* yhat, double* stderr_out) {
+  leaveOneOutFun(theta, theta_n, false, nullptr);
+  std::vector<double> e(n_), s2(n_);
+  rt_->read_slot(0, 7, e.data());
+  rt_->read_slot(0, 8, s2.data());
+  const double sd = std::sqrt(sigma2_);
+  for (int i = 0; i < n_; ++i) {
+    if (yhat) yhat[i] = y_[i] - e[i];
+    if (stderr_out) stderr_out[i] = std::sqrt(s2[i]) * sd;
+  }
+}
+
+double Kriging::leaveOneOut() {
+  if (is_empty_) throw std::runtime_error("model is not fitted");
+  return leaveOneOutFun(theta_.data(), (int)theta_.size(), false, nullptr);
+}
+
 double Kriging::logLikelihood() {
   if (is_empty_) throw std::runtime_error("model is not fitted");
   std::vector<double> g(theta_);
@@ -278,10 +318,11 @@ struct Reparam {
     for (int k = 0; k < d; ++k) theta[k] = log ? std::exp(g[k]) : g[k];
     if (nugget) *alpha = log ? 1.0 + NUGGET_ALPHA_LOWER - std::exp(-g[d]) : g[d];
   }
-  // gradient in gamma of -LL from the gradient in (theta, alpha) of LL
-  void to_gamma_grad(const double* theta, double alpha, const double* gll, double* out) const {
-    for (int k = 0; k < d; ++k) out[k] = -(log ? gll[k] * theta[k] : gll[k]);
-    if (nugget) out[d] = -(log ? gll[d] * (1.0 + NUGGET_ALPHA_LOWER - alpha) : gll[d]);
+  // gradient in gamma of sign * objective from its gradient in (theta,
+  // alpha): sign -1 for the minimised -LL, +1 for LOO
+  void to_gamma_grad(const double* theta, double alpha, const double* gll, double* out, double sign) const {
+    for (int k = 0; k < d; ++k) out[k] = sign * (log ? gll[k] * theta[k] : gll[k]);
+    if (nugget) out[d] = sign * (log ? gll[d] * (1.0 + NUGGET_ALPHA_LOWER - alpha) : gll[d]);
   }
 };
 
@@ -308,7 +349,10 @@ struct FitState : FitJob {
   bool bfgs = false;
   int d = 0, gd = 0;
   bool nug = false;
+  int obj = OBJ_LL;  // LL is maximised (the optimiser sees -LL), LOO minimised
+  int n = 0;
   Reparam rp{0, false, true};
+  double sign() const { return obj == OBJ_LOO ? 1.0 : -1.0; }
   std::vector<double> th_lo, th_up;
   std::vector<StartWorker> W;
   ModelFlags flags;
@@ -362,10 +406,10 @@ struct FitState : FitJob {
         w.error = "Exceed max numerical nugget added to force chol matrix";
         continue;
       }
-      const double f = -res[s].ll;
+      const double f = sign() * res[s].ll;
       double a = 1.0;
       rp.from_gamma(w.opt->x().data(), th.data(), &a);
-      rp.to_gamma_grad(th.data(), a, res[s].grad, gbuf.data());
+      rp.to_gamma_grad(th.data(), a, res[s].grad, gbuf.data(), sign());
       w.opt->tell(f, gbuf);
       if (!w.opt->done()) continue;
       w.gamma = w.opt->x_opt();
@@ -405,7 +449,7 @@ struct FitState : FitJob {
         w.success = false;
         continue;
       }
-      w.objective = -res[s].ll;
+      w.objective = sign() * res[s].ll;
       ++nsucc;
       if (w.objective < min_ofn) {
         min_ofn = w.objective;
@@ -430,8 +474,10 @@ struct FitState : FitJob {
     }
     w.opt = std::make_unique<LBFGSB>(gd, oc.lbfgs_memory);
     w.opt->max_iter = oc.max_iteration;
-    w.opt->pgtol = oc.gradient_tolerance;
-    w.opt->factr = oc.objective_rel_tolerance / 1e-13;
+    // LOO tolerances scaled by 1 / n^2 (Kriging.cpp:1953-1958)
+    const double scale = obj == OBJ_LOO ? 1.0 / ((double)n * n) : 1.0;
+    w.opt->pgtol = oc.gradient_tolerance * scale;
+    w.opt->factr = oc.objective_rel_tolerance / 1e-13 * scale;
     w.opt->start(w.gamma, w.glo, w.gup);
   }
 };
@@ -439,8 +485,11 @@ struct FitState : FitJob {
 std::unique_ptr<FitJob> Kriging::begin_fit(const double* y, const double* X, int n, int d, RegModel regmodel,
                                            bool normalize, const std::string& optim, const std::string& objective,
                                            const Parameters& P) {
-  if (objective != "LL")
-    throw std::invalid_argument("Unsupported objective on this path: " + objective + " (LL)");
+  if (objective != "LL" && objective != "LOO")
+    throw std::invalid_argument("Unsupported objective on this path: " + objective + " (LL, LOO)");
+  // Kriging::make_fit_objective (Kriging.cpp:1471-1474)
+  if (objective == "LOO" && noise_ != NoiseModel::None)
+    throw std::invalid_argument("LOO objective not supported for Nugget/Heterogeneous noise modes");
   int multistart = 1;
   const bool none = optim == "none";
   if (!none) {
@@ -486,6 +535,8 @@ std::unique_ptr<FitJob> Kriging::begin_fit(const double* y, const double* X, int
   st->m = this;
   st->d = d;
   st->nug = noise_ == NoiseModel::Nugget;
+  st->obj = objective == "LOO" ? OBJ_LOO : OBJ_LL;
+  st->n = n;
   st->gd = d + (st->nug ? 1 : 0);
 
   if (none) {
@@ -617,11 +668,14 @@ void Kriging::run_fit_jobs(const std::vector<FitJob*>& jobs_in) {
   for (size_t j = 0; j < jobs.size(); ++j) base[j + 1] = base[j] + jobs[j]->nslots();
   const int total = base.back();
   const bool batched = std::all_of(jobs.begin(), jobs.end(), [](FitState* f) { return f->m->batched_fit_; });
+  const int obj = jobs[0]->obj;
+  for (FitState* f : jobs)
+    if (f->obj != obj) throw std::invalid_argument("run_fit_jobs: all models must share the objective");
   std::vector<EvalRequest> reqs(total);
   std::vector<EvalResult> res;
   auto evaluate = [&](bool want_grad) {
     if (batched) {
-      rt->evaluate(reqs, want_grad, res);
+      rt->evaluate(reqs, want_grad, res, nullptr, obj);
       for (FitState* f : jobs)
         if (f->bfgs) f->m->fit_batches_++;
       return;
@@ -633,7 +687,7 @@ void Kriging::run_fit_jobs(const std::vector<FitJob*>& jobs_in) {
       for (int s = base[j]; s < base[j + 1]; ++s) {
         if (!reqs[s].active) continue;
         one[0] = reqs[s];
-        rt->evaluate(one, want_grad, r1);
+        rt->evaluate(one, want_grad, r1, nullptr, obj);
         res[s] = r1[0];
         jobs[j]->m->fit_batches_++;
       }

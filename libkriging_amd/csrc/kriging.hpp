@@ -90,6 +90,12 @@ class Kriging {
   // nb evaluations at once (rows of gamma, each gamma_n long), one batched pipeline
   void logLikelihoodFunBatch(int nb, const double* gamma, int gamma_n, bool want_grad, double* ll, double* grad);
   double logLikelihood();
+  // leave-one-out objective (Kriging::leaveOneOutFun / leaveOneOut /
+  // leaveOneOutVec, Kriging.cpp:470-485, 669-671); theta has d entries,
+  // noise model none only (as the reference's LOO fit objective)
+  double leaveOneOutFun(const double* theta, int theta_n, bool want_grad, double* grad);
+  void leaveOneOutVec(const double* theta, int theta_n, double* yhat, double* stderr_out);
+  double leaveOneOut();
 
   // Xn: m x d column-major; mean[m], stdev[m] (stdev may be null).
   // cov: m x m; mean_deriv, stdev_deriv: m x d (column-major), when asked

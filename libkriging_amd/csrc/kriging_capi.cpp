@@ -166,6 +166,30 @@ int lk_kriging_log_likelihood_fun_batch(void* ptr, int nb, const double* theta, 
   });
 }
 
+int lk_kriging_leave_one_out_fun(void* ptr, const double* theta, int theta_n, int return_grad, double* loo_out,
+                                 double* grad_out) {
+  return guarded([&] {
+    if (theta == nullptr) throw std::invalid_argument("theta is NULL");
+    std::vector<double> g(theta_n);
+    const double loo = K(ptr)->leaveOneOutFun(theta, theta_n, return_grad != 0, g.data());
+    if (loo_out) *loo_out = loo;
+    if (grad_out && return_grad) std::memcpy(grad_out, g.data(), sizeof(double) * theta_n);
+  });
+}
+
+double lk_kriging_leave_one_out(void* ptr) {
+  double v = 0.0;
+  if (guarded([&] { v = K(ptr)->leaveOneOut(); }) != 0) return std::numeric_limits<double>::quiet_NaN();
+  return v;
+}
+
+int lk_kriging_leave_one_out_vec(void* ptr, const double* theta, int theta_n, double* yhat_out, double* stderr_out) {
+  return guarded([&] {
+    if (theta == nullptr) throw std::invalid_argument("theta is NULL");
+    K(ptr)->leaveOneOutVec(theta, theta_n, yhat_out, stderr_out);
+  });
+}
+
 double lk_kriging_log_likelihood(void* ptr) {
   double v = 0.0;
   if (guarded([&] { v = K(ptr)->logLikelihood(); }) != 0) return std::numeric_limits<double>::quiet_NaN();

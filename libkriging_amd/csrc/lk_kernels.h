@@ -73,6 +73,22 @@ struct SlotOut {
   int pad_[3];
 };
 
+// Columns of Slot::V (npad rows each): [0, p) F*, p y*, p+1 r, p+2 E*, p+3 x,
+// p+4 .. p+6 estimator scratch; the leave-one-out objective adds
+// [p+7, 2p+7) G' (G = R*^-T F*' L^-1, p x n, stored transposed), then
+// sigma2LOO (first diag B), errorsLOO, w, sqrt(a), t = L^-1 w and B w.
+__host__ __device__ inline int v_loo_g(int p) { return p + 7; }
+__host__ __device__ inline int v_loo_s2(int p) { return 2 * p + 7; }
+__host__ __device__ inline int v_loo_err(int p) { return 2 * p + 8; }
+__host__ __device__ inline int v_loo_w(int p) { return 2 * p + 9; }
+__host__ __device__ inline int v_loo_sa(int p) { return 2 * p + 10; }
+__host__ __device__ inline int v_loo_t(int p) { return 2 * p + 11; }
+__host__ __device__ inline int v_loo_bw(int p) { return 2 * p + 12; }
+__host__ __device__ inline int v_cols(int p) { return 2 * p + 13; }
+
+// objective of a batched evaluation
+enum Objective { OBJ_LL = 0, OBJ_LOO = 1 };
+
 // Layout of Slot::red (doubles).
 constexpr int RED_LOGDET = 0;                 // nblk partial log-dets
 __host__ __device__ inline int red_sse_off(int nblk) { return nblk; }          // nblk partial SSE
@@ -158,6 +174,14 @@ hipError_t launch_predict(const PredictArgs& a, hipStream_t stream);
 hipError_t launch_predict_cov(const PredictArgs& a, hipStream_t stream);    // after launch_predict (same chunk)
 hipError_t launch_predict_deriv(const PredictArgs& a, hipStream_t stream);  // after launch_predict (same chunk)
 // plain device copy of `bytes` (a multiple of 16)
+// leave-one-out objective (lk_loo.hip), after the L^-1 path's solves
+hipError_t launch_loo_g(const Batch& b);
+hipError_t launch_loo_value(const Batch& b);
+hipError_t launch_loo_wv(const Batch& b);
+hipError_t launch_loo_bw(const Batch& b);
+hipError_t launch_loo_bs(const Batch& b);
+hipError_t launch_loo_adjoint(const Batch& b);
+hipError_t launch_loo_finalize_grad(const Batch& b);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
 // chains in {1, 2, 4, 8}; out: blocks * 256 doubles
 hipError_t launch_mfma_probe(int blocks, int iters, int chains, double* out, hipStream_t stream);

@@ -429,80 +429,6 @@ __global__ __launch_bounds__(TPB) void k_trtri(const Slot* __restrict__ slots, c
 // (4) triangular products with the explicit inverse.
 // V columns: [0, p) F*, p y*, p+1 r, p+2 E*, p+3 x, p+4.. p+6 estimator scratch
 // ---------------------------------------------------------------------------
-// The matrix-vector kernels stream the L^-1 (and L) triangles once; they
-// run 8 waves per workgroup and issue MV_U independent loads per lane
-// before the first dependent FMA, so a workgroup keeps ~32 KB in flight
-// instead of one 512-byte line per wave.
-constexpr int MV_TPB = 512;
-constexpr int MV_W = MV_TPB / 64;
-constexpr int MV_U = 8;
-
-// out[:, c] = Linv * in[:, c] for rows of block rb (grid (nblk_max, nslots))
-// in columns come from `src` (n rows, ld = n, zero beyond n) ; out to V.
-// Column c of the input is src[c * lds + k] for c < ncols_src and src2[k]
-// for c == ncols_src (when src2 is given), so [F | y] go through in one pass.
-// Wave w sums columns k in batches [w U + j W U, w U + j W U + U); the
-// partial sums of the waves are added in a fixed order (deterministic).
-__device__ inline void lower_matvec_block(const Slot& S, int rb, const double* __restrict__ src, int lds, int nsrc,
-                                          int ncols, double* __restrict__ dst, int ldd, double* __restrict__ part_sq,
-                                          const double* __restrict__ src2 = nullptr) {
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int i0 = rb * NB, ld = S.npad, n = S.n;
-  const int kend = min(i0 + NB, n);  // columns k <= row, and k < n (src is zero beyond n)
-  __shared__ double red[MV_W][4][NB];
-  const double* __restrict__ Wr = S.W + i0 + lane;
-  double ssq = 0.0;
-  for (int c0 = 0; c0 < ncols; c0 += 4) {
-    const int nc = min(4, ncols - c0);
-    // columns past nc re-read column c0 (same cache lines) and are dropped
-    const double* colp[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int cs = c0 + (c < nc ? c : 0);
-      colp[c] = (src2 != nullptr && cs == ncols - 1) ? src2 : src + (size_t)cs * lds;
-    }
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int kb = w * MV_U; kb < kend; kb += MV_W * MV_U) {
-      double lv[MV_U], sv[MV_U][4];
-#pragma unroll
-      for (int u = 0; u < MV_U; ++u) {
-        const int k = min(kb + u, kend - 1);
-        lv[u] = gload(Wr + (size_t)k * ld);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) sv[u][c] = gload(colp[c] + k);
-      }
-#pragma unroll
-      for (int u = 0; u < MV_U; ++u)
-        if (kb + u < kend) {
-#pragma unroll
-          for (int c = 0; c < 4; ++c) acc[c] = fma(lv[u], sv[u][c], acc[c]);
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) red[w][c][lane] = acc[c];
-    __syncthreads();
-    if (w == 0) {
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if (c >= nc) continue;
-        // rows above the diagonal read W's upper part of the diagonal block,
-        // which is exactly zero; rows >= n are identity rows times zero input.
-        double v = red[0][c][lane];
-#pragma unroll
-        for (int q = 1; q < MV_W; ++q) v += red[q][c][lane];
-        dst[(size_t)(c0 + c) * ldd + i0 + lane] = v;
-        ssq = fma(v, v, ssq);
-      }
-    }
-    __syncthreads();
-  }
-  (void)nsrc;
-  if (part_sq != nullptr && w == 0) {
-    const double t = wave_sum(ssq);
-    if (lane == 0) part_sq[rb] = t;
-  }
-}
-
 // [F* | y*] = Linv [F | y]
 __global__ __launch_bounds__(MV_TPB) void k_solve_fy(const Slot* __restrict__ slots,
                                                   const SlotParams* __restrict__ params,

@@ -104,7 +104,7 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
     const int nblk = npad / NB;
     off[s] = total;
     total += align256(sizeof(double) * (size_t)npad * npad) * 2;       // A, W
-    total += align256(sizeof(double) * (size_t)npad * (P.p + 7));       // V
+    total += align256(sizeof(double) * (size_t)npad * v_cols(P.p));     // V
     total += align256(sizeof(double) * red_size(nblk));                 // red
     nblk_max = std::max(nblk_max, nblk);
     dmax = std::max(dmax, P.d);
@@ -132,7 +132,7 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
     S.W = reinterpret_cast<double*>(q);
     q += align256(sizeof(double) * (size_t)npad * npad);
     S.V = reinterpret_cast<double*>(q);
-    q += align256(sizeof(double) * (size_t)npad * (P.p + 7));
+    q += align256(sizeof(double) * (size_t)npad * v_cols(P.p));
     S.red = reinterpret_cast<double*>(q);
     S.n = P.n;
     S.npad = npad;
@@ -167,7 +167,7 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
 // The pipeline as an ordered list of launches, each tagged with the stages
 // it belongs to (bit s = stage s): 0 build, 1 Cholesky, 2 first trailing
 // update, 3 trtri, 4 triangular solves + LL, 5 fused R^-1 / adjoint.
-std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_grad) const {
+std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_grad, int objective) const {
   std::vector<Step> v;
   v.push_back({1u << 0, "build", [B] { return launch_build(B); }});
   // blocked right-looking Cholesky, two levels: panels of P block columns;
@@ -192,7 +192,7 @@ std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_gra
       first_update = false;
     }
   }
-  if (want_grad && rinv_path_) {
+  if (want_grad && rinv_path_ && objective == OBJ_LL) {
     // gradient without L^-1: R^-1 by the backward block sweep (trace part
     // of the adjoint fused), solves by one symmetric product with R^-1
     v.push_back({1u << 4, "norm1_L", [B] { return launch_norm1_L(B); }});
@@ -219,6 +219,20 @@ std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_gra
   v.push_back({1u << 4, "solve_x", [B] { return launch_solve_x(B); }});
   v.push_back({1u << 4, "finalize_ll", [B] { return launch_finalize_ll(B); }});
   v.push_back({1u << 4, "rcond_estimate", [B] { return launch_rcond_estimate(B); }});
+  if (objective == OBJ_LOO) {
+    // leave-one-out value (diag B from the columns of L^-1), then its
+    // gradient: B w, Bs = B diag(sqrt a) into A, Bs Bs' contracted with dR
+    v.push_back({1u << 4, "loo_g", [B] { return launch_loo_g(B); }});
+    v.push_back({1u << 4, "loo_value", [B] { return launch_loo_value(B); }});
+    if (want_grad) {
+      v.push_back({1u << 5, "loo_wv", [B] { return launch_loo_wv(B); }});
+      v.push_back({1u << 5, "loo_bw", [B] { return launch_loo_bw(B); }});
+      v.push_back({1u << 5, "loo_bs", [B] { return launch_loo_bs(B); }});
+      v.push_back({1u << 5, "loo_adjoint", [B] { return launch_loo_adjoint(B); }});
+      v.push_back({0u, "loo_finalize_grad", [B] { return launch_loo_finalize_grad(B); }});
+    }
+    return v;
+  }
   if (want_grad) {
     v.push_back({1u << 5, "lauum_adjoint", [B] { return launch_lauum_adjoint(B); }});
     v.push_back({0u, "finalize_grad", [B] { return launch_finalize_grad(B); }});
@@ -274,7 +288,7 @@ void Runtime::take_stage_time(double* avg_ms, int* count) {
   timed_count_ = 0;
 }
 
-void Runtime::run_pipeline(bool want_grad, BenchMap* bench) {
+void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
   Batch B;
   B.slots = d_slots_;
   B.params = d_params_;
@@ -285,7 +299,7 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench) {
   B.stream = stream_;
   hip_check(hipMemcpyAsync(d_params_, h_params_, sizeof(SlotParams) * nslots_, hipMemcpyHostToDevice, stream_),
             "upload params");
-  const std::vector<Step> steps = pipeline_steps(B, want_grad);
+  const std::vector<Step> steps = pipeline_steps(B, want_grad, objective);
   const auto rng = stage_range(steps, timed_stage_);
   const bool timed = rng.first < rng.second;
   if (bench != nullptr) {
@@ -303,7 +317,7 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench) {
     hipEventDestroy(e0);
     hipEventDestroy(e1);
   } else if (use_graph) {
-    GraphEntry& G = graphs_[want_grad ? 1 : 0];
+    GraphEntry& G = graphs_[2 * objective + (want_grad ? 1 : 0)];
     if (G.version != layout_version_ || G.exec == nullptr || G.timed_stage != timed_stage_) {
       if (G.exec) hip_check(hipGraphExecDestroy(G.exec), "graph exec destroy");
       G.exec = nullptr;
@@ -360,7 +374,8 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench) {
 }
 
 void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std::vector<EvalResult>& res,
-                       BenchMap* bench) {
+                       BenchMap* bench, int objective) {
+  if (objective != OBJ_LL && objective != OBJ_LOO) throw std::invalid_argument("unknown objective");
   const int ns = (int)reqs.size();
   res.assign(ns, EvalResult{});
   if (ns == 0) return;
@@ -395,9 +410,9 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
     Q.want_grad = want_grad;
     Q.gamma_has_alpha = R.gamma_n > P.d;
   }
-  const bool rinv = want_grad && rinv_path_;
+  const bool rinv = want_grad && rinv_path_ && objective == OBJ_LL;
   for (;;) {
-    run_pipeline(want_grad, bench);
+    run_pipeline(want_grad, bench, objective);
     if (rinv) recheck_rcond_bound(ns, bench);
     bool retry = false;
     for (int s = 0; s < ns; ++s) {
@@ -442,6 +457,7 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
     if (!retry) break;
   }
   last_rinv_ = rinv;
+  last_objective_ = objective;
   // leave every slot active for the next call's graph (params are rewritten)
 }
 
@@ -514,6 +530,11 @@ void Runtime::read_slot(int s, int what, double* out) {
       break;
     case 5:
       copy_cols(v.V + (size_t)(v.p + 3) * ld, 1);
+      break;
+    case 7:  // leave-one-out errors
+    case 8:  // sigma2LOO
+      if (last_objective_ != OBJ_LOO) throw std::runtime_error("the last evaluation was not a leave-one-out one");
+      copy_cols(v.V + (size_t)(what == 7 ? v_loo_err(v.p) : v_loo_s2(v.p)) * ld, 1);
       break;
     case 6: {  // R* (p x p, upper), stored by the GLS kernel
       const Slot& S = h_slots_[s];

@@ -88,11 +88,12 @@ class Runtime {
   void remove_problem(int id);
   const DeviceProblem& problem(int id) const;
 
-  // Evaluate LL (and the gradient over gamma) for every request in one
-  // batched pipeline.  Per-request failures (chol still failing after the
-  // retry ladder) are reported in EvalResult::status.
+  // Evaluate LL (objective OBJ_LL) or the leave-one-out objective (OBJ_LOO;
+  // EvalResult::ll then holds the LOO value) and the gradient over gamma for
+  // every request in one batched pipeline.  Per-request failures (chol still
+  // failing after the retry ladder) are reported in EvalResult::status.
   void evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std::vector<EvalResult>& res,
-                BenchMap* bench = nullptr);
+                BenchMap* bench = nullptr, int objective = OBJ_LL);
 
   // Device pointers of slot s after the last evaluate(): the factor L (A,
   // lower, ld = npad), L^-1 (W lower) and the vector block V.
@@ -123,7 +124,7 @@ class Runtime {
 
  private:
   void ensure_slots(const std::vector<EvalRequest>& reqs);
-  void run_pipeline(bool want_grad, BenchMap* bench);
+  void run_pipeline(bool want_grad, BenchMap* bench, int objective = OBJ_LL);
 
  public:
   struct Step {
@@ -133,7 +134,7 @@ class Runtime {
   };
 
  private:
-  std::vector<Step> pipeline_steps(const Batch& B, bool want_grad) const;
+  std::vector<Step> pipeline_steps(const Batch& B, bool want_grad, int objective) const;
   hipGraph_t capture(const std::vector<Step>& v, size_t a, size_t b);
 
   int device_ = 0;
@@ -156,13 +157,14 @@ class Runtime {
   int nslots_ = 0, nblk_max_ = 0, dmax_ = 0;
   uint64_t layout_version_ = 0;
 
-  // graph cache: (layout_version, want_grad) -> exec
+  // graph cache: (layout_version, objective, want_grad) -> exec
   struct GraphEntry {
     uint64_t version = ~0ull;
     int timed_stage = -1;
     hipGraphExec_t exec = nullptr;
   };
-  GraphEntry graphs_[2];
+  GraphEntry graphs_[4];
+  int last_objective_ = OBJ_LL;
   int timed_stage_ = -1;
   // block columns per Cholesky panel (two-level blocking); LKGPU_PANEL_BLOCKS
   int panel_blocks_ = 4;

@@ -33,6 +33,9 @@ N.register("lk_kriging_log_likelihood_fun", ctypes.c_int, [_vp, _dp, ctypes.c_in
 N.register("lk_kriging_log_likelihood_fun_batch", ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.c_int,
                                                                  ctypes.c_int, _dp, _dp])
 N.register("lk_kriging_log_likelihood", ctypes.c_double, [_vp])
+N.register("lk_kriging_leave_one_out_fun", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp])
+N.register("lk_kriging_leave_one_out", ctypes.c_double, [_vp])
+N.register("lk_kriging_leave_one_out_vec", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, _dp])
 for _name in ("lk_kriging_kernel", "lk_kriging_optim", "lk_kriging_objective", "lk_kriging_regmodel",
               "lk_kriging_noise_model"):
     N.register(_name, _cp, [_vp])
@@ -197,6 +200,32 @@ class Kriging:
                                                             int(bool(return_grad)), N.ptr(ll),
                                                             grad.ctypes.data_as(_dp) if grad is not None else None))
         return ll, grad
+
+    def leaveOneOutFun(self, theta, return_grad=False):
+        """(loo, grad) as Kriging::leaveOneOutFun (Kriging.cpp:470-476)"""
+        th = _col(theta)
+        loo = ctypes.c_double()
+        grad = np.zeros(th.size) if return_grad else None
+        N.check(N.lib().lk_kriging_leave_one_out_fun(self._h, N.ptr(th), th.size, int(bool(return_grad)),
+                                                     ctypes.byref(loo), N.ptr(grad) if grad is not None else None))
+        return loo.value, (grad if grad is not None else np.zeros(0))
+
+    def leaveOneOutVec(self, theta):
+        """(yhat, stderr) of the n leave-one-out predictions (Kriging.cpp:478-485)"""
+        th = _col(theta)
+        nn = ctypes.c_int()
+        N.lib().lk_kriging_get_X(self._h, None, ctypes.byref(nn), None)
+        n = nn.value
+        yhat = np.zeros(n)
+        se = np.zeros(n)
+        N.check(N.lib().lk_kriging_leave_one_out_vec(self._h, N.ptr(th), th.size, N.ptr(yhat), N.ptr(se)))
+        return yhat, se
+
+    def leaveOneOut(self):
+        v = N.lib().lk_kriging_leave_one_out(self._h)
+        if np.isnan(v):
+            N.check(-1)
+        return v
 
     def logLikelihood(self):
         v = N.lib().lk_kriging_log_likelihood(self._h)
