@@ -203,6 +203,12 @@ double lk_kriging_log_likelihood(void* ptr);
  * theta has d entries; noise model none only */
 int lk_kriging_leave_one_out_fun(void* ptr, const double* theta, int theta_n, int return_grad, double* loo_out,
                                  double* grad_out);
+/* libkriging_c.h:125-130 (Kriging::logMargPostFun, Kriging.cpp:652-660):
+ * theta has d entries (d + 1 with alpha for the nugget model) */
+int lk_kriging_log_marg_post_fun(void* ptr, const double* theta, int theta_n, int return_grad, double* lmp_out,
+                                 double* grad_out);
+/* libkriging_c.h:134 (Kriging::logMargPost at the fitted parameters) */
+double lk_kriging_log_marg_post(void* ptr);
 /* libkriging_c.h:133 (Kriging::leaveOneOut at the fitted theta) */
 double lk_kriging_leave_one_out(void* ptr);
 /* libkriging_c.h:136 (Kriging::leaveOneOutVec, Kriging.cpp:478-485): n

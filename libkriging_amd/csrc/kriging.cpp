@@ -288,6 +288,35 @@ double Kriging::leaveOneOut() {
   return leaveOneOutFun(theta_.data(), (int)theta_.size(), false, nullptr);
 }
 
+// Kriging::logMargPostFun (Kriging.cpp:652-660): gamma = theta (d; the
+// nugget model appends the current alpha) or [theta, alpha] (d + 1)
+double Kriging::logMargPostFun(const double* gamma, int gamma_n, bool want_grad, double* grad) {
+  if (rt_ == nullptr) throw std::runtime_error("model is not fitted (no data)");
+  if (gamma_n != d_ && !(noise_ == NoiseModel::Nugget && gamma_n == d_ + 1))
+    throw std::invalid_argument("theta must have length " + std::to_string(d_) +
+                                (noise_ == NoiseModel::Nugget ? " or " + std::to_string(d_ + 1) : std::string()));
+  std::vector<double> g(gamma, gamma + gamma_n);
+  if (noise_ == NoiseModel::Nugget && gamma_n == d_) g.push_back(alpha_);
+  std::vector<EvalRequest> reqs(1);
+  reqs[0].problem = problem_;
+  reqs[0].gamma = g.data();
+  reqs[0].gamma_n = (int)g.size();
+  reqs[0].flags = flags_for_eval();
+  std::vector<EvalResult> res;
+  rt_->evaluate(reqs, want_grad, res, nullptr, OBJ_LMP);
+  if (res[0].status != EVAL_OK)
+    throw std::runtime_error("[ERROR] Exceed max numerical nugget (" + std::to_string(res[0].inc) +
+                             " x 1e-10) added to force chol matrix");
+  if (want_grad)
+    for (int k = 0; k < gamma_n; ++k) grad[k] = res[0].grad[k];
+  return res[0].ll;
+}
+
+double Kriging::logMargPost() {
+  if (is_empty_) throw std::runtime_error("model is not fitted");
+  return logMargPostFun(theta_.data(), (int)theta_.size(), false, nullptr);
+}
+
 double Kriging::logLikelihood() {
   if (is_empty_) throw std::runtime_error("model is not fitted");
   std::vector<double> g(theta_);
@@ -349,7 +378,7 @@ struct FitState : FitJob {
   bool bfgs = false;
   int d = 0, gd = 0;
   bool nug = false;
-  int obj = OBJ_LL;  // LL is maximised (the optimiser sees -LL), LOO minimised
+  int obj = OBJ_LL;  // LL and LMP are maximised (the optimiser sees -f), LOO minimised
   int n = 0;
   Reparam rp{0, false, true};
   double sign() const { return obj == OBJ_LOO ? 1.0 : -1.0; }
@@ -485,8 +514,8 @@ struct FitState : FitJob {
 std::unique_ptr<FitJob> Kriging::begin_fit(const double* y, const double* X, int n, int d, RegModel regmodel,
                                            bool normalize, const std::string& optim, const std::string& objective,
                                            const Parameters& P) {
-  if (objective != "LL" && objective != "LOO")
-    throw std::invalid_argument("Unsupported objective on this path: " + objective + " (LL, LOO)");
+  if (objective != "LL" && objective != "LOO" && objective != "LMP")
+    throw std::invalid_argument("Unsupported objective on this path: " + objective + " (LL, LOO, LMP)");
   // Kriging::make_fit_objective (Kriging.cpp:1471-1474)
   if (objective == "LOO" && noise_ != NoiseModel::None)
     throw std::invalid_argument("LOO objective not supported for Nugget/Heterogeneous noise modes");
@@ -535,7 +564,7 @@ std::unique_ptr<FitJob> Kriging::begin_fit(const double* y, const double* X, int
   st->m = this;
   st->d = d;
   st->nug = noise_ == NoiseModel::Nugget;
-  st->obj = objective == "LOO" ? OBJ_LOO : OBJ_LL;
+  st->obj = objective == "LOO" ? OBJ_LOO : objective == "LMP" ? OBJ_LMP : OBJ_LL;
   st->n = n;
   st->gd = d + (st->nug ? 1 : 0);
 
@@ -934,6 +963,7 @@ void Kriging::commit_apply(const std::vector<double>& theta, double alpha, int s
       // BFGS commit (Kriging.cpp:2174-2189)
       if (est_nugget_) {
         sigma2_ = alpha * total_var;
+        if (objective_ == "LMP") sigma2_ = sigma2_ * n / (n - p - 2);
         nugget_ = sigma2_ / alpha - sigma2_;
       } else {
         sigma2_ = nugget_ * alpha / (1.0 - alpha);
@@ -943,6 +973,8 @@ void Kriging::commit_apply(const std::vector<double>& theta, double alpha, int s
     }
   } else if (est_sigma2_) {
     sigma2_ = total_var;
+    // BFGS commit with the LMP objective (Kriging.cpp:2192-2195)
+    if (est_theta_ && objective_ == "LMP") sigma2_ = sse / (n - p);
   }
   is_empty_ = false;
 }

@@ -96,6 +96,10 @@ class Kriging {
   double leaveOneOutFun(const double* theta, int theta_n, bool want_grad, double* grad);
   void leaveOneOutVec(const double* theta, int theta_n, double* yhat, double* stderr_out);
   double leaveOneOut();
+  // log marginal posterior (Kriging::logMargPostFun / logMargPost,
+  // Kriging.cpp:488-672); gamma as logLikelihoodFun
+  double logMargPostFun(const double* gamma, int gamma_n, bool want_grad, double* grad);
+  double logMargPost();
 
   // Xn: m x d column-major; mean[m], stdev[m] (stdev may be null).
   // cov: m x m; mean_deriv, stdev_deriv: m x d (column-major), when asked

@@ -177,6 +177,23 @@ int lk_kriging_leave_one_out_fun(void* ptr, const double* theta, int theta_n, in
   });
 }
 
+int lk_kriging_log_marg_post_fun(void* ptr, const double* theta, int theta_n, int return_grad, double* lmp_out,
+                                 double* grad_out) {
+  return guarded([&] {
+    if (theta == nullptr) throw std::invalid_argument("theta is NULL");
+    std::vector<double> g(theta_n + 1);
+    const double v = K(ptr)->logMargPostFun(theta, theta_n, return_grad != 0, g.data());
+    if (lmp_out) *lmp_out = v;
+    if (grad_out && return_grad) std::memcpy(grad_out, g.data(), sizeof(double) * theta_n);
+  });
+}
+
+double lk_kriging_log_marg_post(void* ptr) {
+  double v = 0.0;
+  if (guarded([&] { v = K(ptr)->logMargPost(); }) != 0) return std::numeric_limits<double>::quiet_NaN();
+  return v;
+}
+
 double lk_kriging_leave_one_out(void* ptr) {
   double v = 0.0;
   if (guarded([&] { v = K(ptr)->leaveOneOut(); }) != 0) return std::numeric_limits<double>::quiet_NaN();

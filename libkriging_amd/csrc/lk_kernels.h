@@ -55,7 +55,7 @@ struct SlotParams {
   int rcond_check;
   int want_grad;
   int gamma_has_alpha; // gradient also over alpha
-  int pad_;
+  int objective;       // OBJ_LL, OBJ_LOO or OBJ_LMP (batch-wide)
 };
 
 // Per-slot outputs (downloaded after each batch).
@@ -69,6 +69,7 @@ struct SlotOut {
   double rcond2;     // (1 / (|L|_1 |L^-1|_1))^2, or the estimator's value
   double grad[DMAX + 1];
   double beta[PMAX];
+  double logdet_rstar; // sum log R*_aa (log-marginal-posterior objective)
   int status;
   int pad_[3];
 };
@@ -87,7 +88,7 @@ __host__ __device__ inline int v_loo_bw(int p) { return 2 * p + 12; }
 __host__ __device__ inline int v_cols(int p) { return 2 * p + 13; }
 
 // objective of a batched evaluation
-enum Objective { OBJ_LL = 0, OBJ_LOO = 1 };
+enum Objective { OBJ_LL = 0, OBJ_LOO = 1, OBJ_LMP = 2, OBJ_COUNT = 3 };
 
 // Layout of Slot::red (doubles).
 constexpr int RED_LOGDET = 0;                 // nblk partial log-dets
@@ -174,6 +175,9 @@ hipError_t launch_predict(const PredictArgs& a, hipStream_t stream);
 hipError_t launch_predict_cov(const PredictArgs& a, hipStream_t stream);    // after launch_predict (same chunk)
 hipError_t launch_predict_deriv(const PredictArgs& a, hipStream_t stream);  // after launch_predict (same chunk)
 // plain device copy of `bytes` (a multiple of 16)
+// log-marginal-posterior gradient: the fused R^-1 / adjoint pass with
+// B = R^-1 - G'G in place of R^-1 (after launch_loo_g)
+hipError_t launch_lmp_adjoint(const Batch& b);
 // leave-one-out objective (lk_loo.hip), after the L^-1 path's solves
 hipError_t launch_loo_g(const Batch& b);
 hipError_t launch_loo_value(const Batch& b);

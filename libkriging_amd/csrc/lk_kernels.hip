@@ -498,8 +498,12 @@ __global__ __launch_bounds__(TPB) void k_gls(const Slot* __restrict__ slots, con
       out[b].beta[a] = z[a];
     }
     double* rs = S.red + red_rstar_off(S.nblk);
-    for (int c = 0; c < p; ++c)
+    double ldr = 0.0;
+    for (int c = 0; c < p; ++c) {
+      ldr += log(G[c + c * PMAX]);
       for (int a = 0; a < p; ++a) rs[a + c * p] = (a <= c) ? G[a + c * PMAX] : 0.0;
+    }
+    out[b].logdet_rstar = ldr;
     if (!ok) out[b].status = ST_CHOL_FAILED;
   }
   __syncthreads();
@@ -625,6 +629,21 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
   if (P.rcond_check && !(O.rcond2 >= P.min_rcond) && O.status == ST_OK)
     O.status = rinv_path ? ST_RCOND_BOUND : ST_RCOND_SUSPECT;
   double ll, s2g;
+  if (P.objective == OBJ_LMP) {
+    // log marginal likelihood of Kriging::_logMargPost (Kriging.cpp:560-583),
+    // S_2 = y'R^-1 y - y'P y = SSE; the prior term is added by the host
+    const double nmp = n - S.p;
+    if (S.noise == NOISE_NUGGET) {
+      if (P.est_sigma2 && P.est_nugget) s2g = sse / nmp;
+      else if (P.est_sigma2 || P.est_nugget) s2g = P.sigma2 / P.alpha;
+      else s2g = P.sigma2 + P.nugget;
+    } else {
+      s2g = P.est_sigma2 ? sse / nmp : P.sigma2;
+    }
+    O.ll = -logdet - O.logdet_rstar - 0.5 * nmp * log(s2g * nmp);
+    O.sigma2 = s2g;
+    return;
+  }
   if (S.noise == NOISE_NUGGET) {
     const double a = P.alpha;
     double s2 = P.sigma2, nug = P.nugget;
@@ -662,9 +681,13 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
 //   a1   += x_i x_j rho_ij        a2   += Rinv_ij rho_ij
 // and diagonal elements add x_i^2 and Rinv_ii.  Partials per tile go to
 // red[grad_off + tile * GRAD_STRIDE]: [s1 (DT) | s2 (DT) | a1 a2 dg1 dg2].
+// LMP: the log-marginal-posterior gradient (KrigingImpl.cpp:887-921) has the
+// same contraction with B = R^-1 - R^-1 F (F'R^-1 F)^-1 F'R^-1 = R^-1 - G'G in
+// place of R^-1 (G = R*^-T F*' L^-1, columns of V from the leave-one-out
+// pass k_loo_g), subtracted from each tile before the contraction.
 // grid (nblk_max (nblk_max + 1) / 2, nslots), 256 threads
 // ---------------------------------------------------------------------------
-template <int DT>
+template <int DT, bool LMP>
 __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ slots,
                                                        const SlotParams* __restrict__ params,
                                                        SlotOut* __restrict__ out) {
@@ -704,6 +727,15 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
     sxi[tid] = xv[i0 + tid];
     sxj[tid] = xv[j0 + tid];
   }
+  __shared__ double sG[LMP ? 2 * PMAX * NB : 1];
+  if (LMP) {
+    const double* Gt = S.V + (size_t)v_loo_g(S.p) * ld;
+    for (int q = tid; q < S.p * NB; q += TPB) {
+      const int a = q / NB, r = q % NB;
+      sG[q] = Gt[(size_t)a * ld + i0 + r];
+      sG[PMAX * NB + q] = Gt[(size_t)a * ld + j0 + r];
+    }
+  }
   __syncthreads();
 
   double it_[DT], it2[DT];
@@ -731,8 +763,10 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
       for (int e = 0; e < 4; ++e) {
         const int r = acc_row(lane, wr, bi), c = acc_col(lane, wc, bj, e);
         const int gi = i0 + r, gj = j0 + c;
-        const double rinv = acc[bi][bj][e];
+        double rinv = acc[bi][bj][e];
         if (gi >= n || gj > gi) continue;
+        if (LMP)
+          for (int a = 0; a < S.p; ++a) rinv = fma(-sG[a * NB + r], sG[PMAX * NB + a * NB + c], rinv);
         const double xx = sxi[r] * sxj[c];
         if (gi == gj) {
           dg1 += xx;
@@ -863,7 +897,11 @@ __global__ __launch_bounds__(TPB) void k_finalize_grad(const Slot* __restrict__ 
     const double a1 = tot[2 * DMAX], a2 = tot[2 * DMAX + 1], dg1 = tot[2 * DMAX + 2], dg2 = tot[2 * DMAX + 3];
     const double a = P.alpha;
     double ga;
-    if (P.est_sigma2 && P.est_nugget) {
+    if (P.objective == OBJ_LMP) {
+      // -tr(Wb)/2 + y'Wb'Q_output/(2 sigma2), Wb = D B, D = R / alpha with
+      // zero diagonal (Kriging.cpp:612-618); the prior term is the host's
+      ga = a1 / s2g - a2;
+    } else if (P.est_sigma2 && P.est_nugget) {
       // -0.5 * (-(x' D x) / var + <Rinv, D>), D = R / alpha with zero diagonal
       ga = -0.5 * (-(2.0 * a1) / s2g + 2.0 * a2);
     } else if (P.est_sigma2 && !P.est_nugget) {
@@ -1448,19 +1486,22 @@ hipError_t launch_finalize_ll(const Batch& B, bool rinv_path) {
                      B.nslots, rinv_path ? 1 : 0);
   return hipGetLastError();
 }
-hipError_t launch_lauum_adjoint(const Batch& B) {
+template <bool LMP>
+static hipError_t launch_lauum_adjoint_t(const Batch& B) {
   dim3 grid(tri(B.nblk_max), B.nslots);
   const int dmax_batch = B.dmax;
   if (dmax_batch <= 4)
-    hipLaunchKernelGGL(k_lauum_adjoint<4>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+    hipLaunchKernelGGL((k_lauum_adjoint<4, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   else if (dmax_batch <= 8)
-    hipLaunchKernelGGL(k_lauum_adjoint<8>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+    hipLaunchKernelGGL((k_lauum_adjoint<8, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   else if (dmax_batch <= 16)
-    hipLaunchKernelGGL(k_lauum_adjoint<16>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+    hipLaunchKernelGGL((k_lauum_adjoint<16, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   else
-    hipLaunchKernelGGL(k_lauum_adjoint<32>, grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+    hipLaunchKernelGGL((k_lauum_adjoint<32, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   return hipGetLastError();
 }
+hipError_t launch_lauum_adjoint(const Batch& B) { return launch_lauum_adjoint_t<false>(B); }
+hipError_t launch_lmp_adjoint(const Batch& B) { return launch_lauum_adjoint_t<true>(B); }
 hipError_t launch_finalize_grad(const Batch& B) {
   hipLaunchKernelGGL(k_finalize_grad, dim3(B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, B.nslots);
   return hipGetLastError();

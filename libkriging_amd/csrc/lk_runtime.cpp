@@ -57,6 +57,15 @@ int Runtime::add_problem(const double* X, int n, int d, const double* y, const d
   P.d = d;
   P.p = p;
   P.kernel = kernel;
+  P.range.assign(d, 0.0);
+  for (int k = 0; k < d; ++k) {
+    double mn = X[(size_t)k * n], mx = mn;
+    for (int i = 1; i < n; ++i) {
+      mn = std::min(mn, X[i + (size_t)k * n]);
+      mx = std::max(mx, X[i + (size_t)k * n]);
+    }
+    P.range[k] = mx - mn;
+  }
   P.noise = noise;
   hip_check(hipMalloc(&P.X, sizeof(double) * (size_t)n * d), "hipMalloc X");
   hip_check(hipMalloc(&P.y, sizeof(double) * (size_t)n), "hipMalloc y");
@@ -233,6 +242,16 @@ std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_gra
     }
     return v;
   }
+  if (objective == OBJ_LMP) {
+    // log marginal posterior: the value comes out of finalize_ll; its
+    // gradient is the likelihood's contraction with B = R^-1 - G'G
+    if (want_grad) {
+      v.push_back({1u << 5, "loo_g", [B] { return launch_loo_g(B); }});
+      v.push_back({1u << 5, "lmp_adjoint", [B] { return launch_lmp_adjoint(B); }});
+      v.push_back({0u, "finalize_grad", [B] { return launch_finalize_grad(B); }});
+    }
+    return v;
+  }
   if (want_grad) {
     v.push_back({1u << 5, "lauum_adjoint", [B] { return launch_lauum_adjoint(B); }});
     v.push_back({0u, "finalize_grad", [B] { return launch_finalize_grad(B); }});
@@ -317,6 +336,7 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
     hipEventDestroy(e0);
     hipEventDestroy(e1);
   } else if (use_graph) {
+    if (objective < 0 || objective >= OBJ_COUNT) throw std::invalid_argument("unknown objective");
     GraphEntry& G = graphs_[2 * objective + (want_grad ? 1 : 0)];
     if (G.version != layout_version_ || G.exec == nullptr || G.timed_stage != timed_stage_) {
       if (G.exec) hip_check(hipGraphExecDestroy(G.exec), "graph exec destroy");
@@ -375,7 +395,7 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
 
 void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std::vector<EvalResult>& res,
                        BenchMap* bench, int objective) {
-  if (objective != OBJ_LL && objective != OBJ_LOO) throw std::invalid_argument("unknown objective");
+  if (objective < 0 || objective >= OBJ_COUNT) throw std::invalid_argument("unknown objective");
   const int ns = (int)reqs.size();
   res.assign(ns, EvalResult{});
   if (ns == 0) return;
@@ -409,6 +429,7 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
     Q.rcond_check = R.flags.rcond_check >= 0 ? R.flags.rcond_check : chol.rcond_check;
     Q.want_grad = want_grad;
     Q.gamma_has_alpha = R.gamma_n > P.d;
+    Q.objective = objective;
   }
   const bool rinv = want_grad && rinv_path_ && objective == OBJ_LL;
   for (;;) {
@@ -458,7 +479,68 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
   }
   last_rinv_ = rinv;
   last_objective_ = objective;
+  if (objective == OBJ_LMP) add_lmp_prior(reqs, res, want_grad);
   // leave every slot active for the next call's graph (params are rewritten)
+}
+
+// The approximate reference prior of the LMP objective (Kriging.cpp:585-598,
+// 600-606, 618): a = 0.2, b = (a + d) / n^(1/d), CL = range(X) / n^(1/d),
+// t = sum CL / theta (+ (1 - alpha) / alpha for the nugget model), prior
+// -b t + a log t.  With sigma2 fixed the reference's gradient is a forward
+// difference (step 1e-6 in theta) of the whole objective, reproduced here by
+// d more batched evaluations; the alpha component is then 0.
+void Runtime::add_lmp_prior(const std::vector<EvalRequest>& reqs, std::vector<EvalResult>& res,
+                            bool want_grad) const {
+  const int ns = (int)reqs.size();
+  for (int s = 0; s < ns; ++s) {
+    EvalResult& E = res[s];
+    if (!reqs[s].active || E.status != EVAL_OK) continue;
+    const DeviceProblem& P = problem(reqs[s].problem);
+    const int d = P.d;
+    const double n = P.n;
+    const double a = 0.2, b = 1.0 / std::pow(n, 1.0 / d) * (a + d);
+    const bool nug = P.noise == NOISE_NUGGET;
+    const double alpha = h_params_[s].alpha;
+    double t = nug ? (1.0 - alpha) / alpha : 0.0;
+    for (int k = 0; k < d; ++k) t += P.range[k] / std::pow(n, 1.0 / d) / reqs[s].gamma[k];
+    E.ll += -b * t + a * std::log(t);
+    if (!want_grad) continue;
+    for (int k = 0; k < d; ++k) {
+      const double cl = P.range[k] / std::pow(n, 1.0 / d), th = reqs[s].gamma[k];
+      E.grad[k] -= (a * cl / t - b * cl) / (th * th);
+    }
+    if (nug && reqs[s].gamma_n > d) E.grad[d] -= (a / t - b) / (alpha * alpha);
+  }
+  if (!want_grad) return;
+  // fixed sigma2: forward differences of the objective (Kriging.cpp:634-646)
+  std::vector<int> fd;
+  for (int s = 0; s < ns; ++s)
+    if (reqs[s].active && res[s].status == EVAL_OK && !reqs[s].flags.est_sigma2) fd.push_back(s);
+  if (fd.empty()) return;
+  Runtime* self = const_cast<Runtime*>(this);
+  int dmax = 0;
+  for (int s : fd) dmax = std::max(dmax, problem(reqs[s].problem).d);
+  std::vector<std::vector<double>> gam(ns);
+  std::vector<EvalRequest> r2(reqs);
+  std::vector<EvalResult> res2;
+  const double eps = 1e-6;
+  for (int k = 0; k < dmax; ++k) {
+    for (int s = 0; s < ns; ++s) {
+      r2[s].active = false;
+      if (std::find(fd.begin(), fd.end(), s) == fd.end() || k >= problem(reqs[s].problem).d) continue;
+      gam[s].assign(reqs[s].gamma, reqs[s].gamma + reqs[s].gamma_n);
+      gam[s][k] += eps;
+      r2[s].gamma = gam[s].data();
+      r2[s].active = true;
+    }
+    self->evaluate(r2, false, res2, nullptr, OBJ_LMP);
+    for (int s : fd)
+      if (k < problem(reqs[s].problem).d) res[s].grad[k] = (res2[s].ll - res[s].ll) / eps;
+  }
+  for (int s : fd) {
+    const DeviceProblem& P = problem(reqs[s].problem);
+    if (P.noise == NOISE_NUGGET && reqs[s].gamma_n > P.d) res[s].grad[P.d] = 0.0;
+  }
 }
 
 // Slots of an R^-1-sweep evaluation whose cheap rcond bound fell below the

@@ -71,6 +71,7 @@ struct DeviceProblem {
   double* X = nullptr;
   double* y = nullptr;
   double* F = nullptr;
+  std::vector<double> range;  // max - min of each input column (prior of the LMP objective)
 };
 
 // Per-evaluation timings filled when a bench map is requested (Bench::toc
@@ -88,9 +89,10 @@ class Runtime {
   void remove_problem(int id);
   const DeviceProblem& problem(int id) const;
 
-  // Evaluate LL (objective OBJ_LL) or the leave-one-out objective (OBJ_LOO;
-  // EvalResult::ll then holds the LOO value) and the gradient over gamma for
-  // every request in one batched pipeline.  Per-request failures (chol still
+  // Evaluate LL (objective OBJ_LL), the leave-one-out objective (OBJ_LOO)
+  // or the log marginal posterior (OBJ_LMP) -- EvalResult::ll holds the
+  // objective's value -- and its gradient over gamma for every request in
+  // one batched pipeline.  Per-request failures (chol still
   // failing after the retry ladder) are reported in EvalResult::status.
   void evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std::vector<EvalResult>& res,
                 BenchMap* bench = nullptr, int objective = OBJ_LL);
@@ -124,6 +126,7 @@ class Runtime {
 
  private:
   void ensure_slots(const std::vector<EvalRequest>& reqs);
+  void add_lmp_prior(const std::vector<EvalRequest>& reqs, std::vector<EvalResult>& res, bool want_grad) const;
   void run_pipeline(bool want_grad, BenchMap* bench, int objective = OBJ_LL);
 
  public:
@@ -163,7 +166,7 @@ class Runtime {
     int timed_stage = -1;
     hipGraphExec_t exec = nullptr;
   };
-  GraphEntry graphs_[4];
+  GraphEntry graphs_[2 * OBJ_COUNT];
   int last_objective_ = OBJ_LL;
   int timed_stage_ = -1;
   // block columns per Cholesky panel (two-level blocking); LKGPU_PANEL_BLOCKS

@@ -35,6 +35,8 @@ N.register("lk_kriging_log_likelihood_fun_batch", ctypes.c_int, [_vp, ctypes.c_i
 N.register("lk_kriging_log_likelihood", ctypes.c_double, [_vp])
 N.register("lk_kriging_leave_one_out_fun", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp])
 N.register("lk_kriging_leave_one_out", ctypes.c_double, [_vp])
+N.register("lk_kriging_log_marg_post_fun", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp])
+N.register("lk_kriging_log_marg_post", ctypes.c_double, [_vp])
 N.register("lk_kriging_leave_one_out_vec", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, _dp])
 for _name in ("lk_kriging_kernel", "lk_kriging_optim", "lk_kriging_objective", "lk_kriging_regmodel",
               "lk_kriging_noise_model"):
@@ -220,6 +222,21 @@ class Kriging:
         se = np.zeros(n)
         N.check(N.lib().lk_kriging_leave_one_out_vec(self._h, N.ptr(th), th.size, N.ptr(yhat), N.ptr(se)))
         return yhat, se
+
+    def logMargPostFun(self, theta, return_grad=False):
+        """(lmp, grad) as Kriging::logMargPostFun (Kriging.cpp:652-660)"""
+        th = _col(theta)
+        v = ctypes.c_double()
+        grad = np.zeros(th.size) if return_grad else None
+        N.check(N.lib().lk_kriging_log_marg_post_fun(self._h, N.ptr(th), th.size, int(bool(return_grad)),
+                                                     ctypes.byref(v), N.ptr(grad) if grad is not None else None))
+        return v.value, (grad if grad is not None else np.zeros(0))
+
+    def logMargPost(self):
+        v = N.lib().lk_kriging_log_marg_post(self._h)
+        if np.isnan(v):
+            N.check(-1)
+        return v
 
     def leaveOneOut(self):
         v = N.lib().lk_kriging_leave_one_out(self._h)

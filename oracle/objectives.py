@@ -1,6 +1,7 @@
 """CPU restatement of the leave-one-out objective (Kriging::_leaveOneOut,
-src/lib/Kriging.cpp:353-468) for small n, in numpy on top of the C oracle's
-covariance functions.
+src/lib/Kriging.cpp:353-468) and of the log marginal posterior
+(Kriging::_logMargPost, Kriging.cpp:488-650) for small n, in numpy on top of
+the C oracle's covariance functions.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker.  The product
 package (libkriging_amd) never imports it.
@@ -12,6 +13,10 @@ Pinned by (tests/test_loo_oracle.py):
     fits on n - 1 points each (Dubrule's identity), for every kernel and
     trend;
   - central finite differences of the value for every gradient component.
+The log marginal posterior has no reference fixture: its value is pinned by
+the slogdet closed form of the marginal likelihood and the y-scaling
+identity, its gradient by finite differences; agreement with the
+reference's own numbers is parity unpinned.
 """
 import numpy as np
 
@@ -107,3 +112,84 @@ def leave_one_out_bruteforce(X, y, kernel, theta, regmodel="constant"):
         pred = F[i] @ beta + Ri_r @ (yk - Fk @ beta)
         err[i] = y[i] - pred
     return err, var
+
+
+def log_marg_post(X, y, kernel, theta, regmodel="constant", alpha=None, est_sigma2=True, sigma2=None,
+                  est_nugget=True, nugget=None, want_grad=True):
+    """(lmp, grad) following Kriging::_logMargPost (Kriging.cpp:488-650) and
+    KrigingImpl::compute_lmp_theta_ans (KrigingImpl.cpp:887-921).  alpha is
+    None for the noise-free model, else the nugget model's alpha (R =
+    alpha C + (1 - alpha) I) and grad has d + 1 entries.  sigma2 / nugget are
+    the model's fixed values (used when not estimated)."""
+    X = np.asarray(X, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64).reshape(-1)
+    theta = np.asarray(theta, dtype=np.float64).reshape(-1)
+    n, d = X.shape
+    nug = alpha is not None
+    a_ = float(alpha) if nug else 1.0
+    C = O.corr_mat(kernel, X, X, theta)
+    R = a_ * C
+    np.fill_diagonal(R, 1.0)
+    L, _, _ = O.safe_chol_lower(R)
+    F = O.trend_matrix(X, regmodel)
+    p = F.shape[1]
+    Fs = np.linalg.solve(L, F)
+    ys = np.linalg.solve(L, y)
+    Rinv_X = np.linalg.solve(L.T, Fs)
+    if p > 0:
+        XtRiX = F.T @ Rinv_X
+        LX, _, _ = O.safe_chol_lower(XtRiX)
+        P = Rinv_X @ np.linalg.solve(LX.T, np.linalg.solve(LX, Rinv_X.T))
+        logdet_lx = np.sum(np.log(np.diag(LX)))
+    else:
+        P = np.zeros((n, n))
+        logdet_lx = 0.0
+    yt_Rinv = np.linalg.solve(L.T, ys)
+    S_2 = yt_Rinv @ y - y @ P @ y
+    if nug:
+        if est_sigma2 and est_nugget:
+            s2 = S_2 / (n - p)
+        elif est_sigma2 or est_nugget:
+            s2 = sigma2 / a_
+        else:
+            s2 = sigma2 + nugget
+    elif est_sigma2:
+        s2 = S_2 / (n - p)
+    else:
+        s2 = sigma2
+    log_S_2 = np.log(s2 * (n - p))
+    lml = -np.sum(np.log(np.diag(L))) - logdet_lx - (n - p) / 2.0 * log_S_2
+    a = 0.2
+    b = 1.0 / n ** (1.0 / d) * (a + d)
+    CL = (X.max(axis=0) - X.min(axis=0)) / n ** (1.0 / d)
+    t = np.sum(CL / theta) + ((1.0 - a_) / a_ if nug else 0.0)
+    lmp = float(lml + (-b * t + a * np.log(t)))
+    if not want_grad:
+        return lmp, None
+    g = np.zeros(d + (1 if nug else 0))
+    if est_sigma2:
+        Q_output = yt_Rinv - P @ y
+        for k in range(d):
+            G = np.zeros((n, n))
+            for i in range(n):
+                for j in range(i):
+                    v = R[i, j] * O.dlncov_dtheta(kernel, X[i] - X[j], theta)[k]
+                    G[i, j] = G[j, i] = v
+            Wb = np.linalg.solve(L.T, np.linalg.solve(L, G)).T - G @ P
+            ans = -np.trace(Wb) / 2.0 + y @ Wb.T @ Q_output / (2.0 * s2)
+            g[k] = ans - (a * CL[k] / t - b * CL[k]) / theta[k] ** 2
+        if nug:
+            if est_sigma2 or est_nugget:
+                Gd = R / a_
+                np.fill_diagonal(Gd, 0.0)
+                Wb = np.linalg.solve(L.T, np.linalg.solve(L, Gd)).T - Gd @ P
+                ans = -np.trace(Wb) / 2.0 + y @ Wb.T @ Q_output / (2.0 * s2)
+                g[d] = ans - (a / t - b) / a_ ** 2
+    else:
+        eps = 1e-6
+        for k in range(d):
+            te = theta.copy()
+            te[k] += eps
+            g[k] = (log_marg_post(X, y, kernel, te, regmodel, alpha, est_sigma2, sigma2, est_nugget, nugget,
+                                  want_grad=False)[0] - lmp) / eps
+    return lmp, g

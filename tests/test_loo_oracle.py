@@ -69,3 +69,90 @@ def test_loo_gradient_vs_finite_differences(kernel, regmodel):
         fm = OB.leave_one_out(X, y, kernel, tm, regmodel=regmodel, want_grad=False)[0]
         fd[k] = (fp - fm) / (2 * h)
     np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-9 * np.max(np.abs(fd)))
+
+
+# --- log marginal posterior (Kriging::_logMargPost) -------------------------
+# No reference fixture holds LMP values: the value is pinned by a closed form
+# of the marginal likelihood (slogdet, no Cholesky) and a scaling identity,
+# the gradient by finite differences of the value.
+
+def _lml_closed_form(X, y, kernel, theta, regmodel):
+    """-1/2 log|R| - 1/2 log|F'R^-1 F| - (n-p)/2 log(y'(R^-1 - R^-1 F (F'R^-1F)^-1 F'R^-1) y)"""
+    from oracle import oracle as O
+    n, d = X.shape
+    R = O.corr_mat(kernel, X, X, theta)
+    np.fill_diagonal(R, 1.0)
+    F = O.trend_matrix(X, regmodel)
+    p = F.shape[1]
+    Ri = np.linalg.inv(R)
+    M = F.T @ Ri @ F
+    Q = Ri - Ri @ F @ np.linalg.solve(M, F.T @ Ri) if p else Ri
+    ld_M = np.linalg.slogdet(M)[1] if p else 0.0
+    lml = -0.5 * np.linalg.slogdet(R)[1] - 0.5 * ld_M - (n - p) / 2.0 * np.log(y @ Q @ y)
+    a = 0.2
+    b = (a + d) / n ** (1.0 / d)
+    CL = (X.max(axis=0) - X.min(axis=0)) / n ** (1.0 / d)
+    t = np.sum(CL / theta)
+    return lml - b * t + a * np.log(t)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("regmodel", ["none", "constant", "linear"])
+def test_lmp_closed_form(kernel, regmodel):
+    X, y = _design(30, 2, 4)
+    theta = np.array([0.4, 0.7])
+    lmp, _ = OB.log_marg_post(X, y, kernel, theta, regmodel=regmodel, want_grad=False)
+    from oracle import oracle as O
+    R = O.corr_mat(kernel, X, X, theta)
+    np.fill_diagonal(R, 1.0)
+    # the closed form inverts R explicitly: agreement to cond(R) eps
+    tol = max(1e-12, 10 * np.linalg.cond(R) * np.finfo(float).eps)
+    assert rel_err(lmp, _lml_closed_form(X, y, kernel, theta, regmodel)) < tol
+
+
+def test_lmp_scaling_identity():
+    # y -> c y moves S_2 by c^2: LMP drops by (n - p) log c
+    X, y = _design(30, 2, 4)
+    theta = np.array([0.4, 0.7])
+    l1, _ = OB.log_marg_post(X, y, "matern5_2", theta, want_grad=False)
+    l2, _ = OB.log_marg_post(X, 3.0 * y, "matern5_2", theta, want_grad=False)
+    assert abs((l1 - l2) - (30 - 1) * np.log(3.0)) < 1e-9
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("regmodel", ["constant", "linear"])
+def test_lmp_gradient_vs_finite_differences(kernel, regmodel):
+    X, y = _design(20, 3, 7)
+    theta = np.array([0.5, 0.3, 0.8])
+    _, g = OB.log_marg_post(X, y, kernel, theta, regmodel=regmodel)
+    fd = np.zeros(3)
+    for k in range(3):
+        h = 1e-6 * theta[k]
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (OB.log_marg_post(X, y, kernel, tp, regmodel=regmodel, want_grad=False)[0]
+                 - OB.log_marg_post(X, y, kernel, tm, regmodel=regmodel, want_grad=False)[0]) / (2 * h)
+    np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-8 * np.max(np.abs(fd)))
+
+
+@pytest.mark.parametrize("kernel", ["gauss", "matern3_2"])
+def test_lmp_nugget_gradient_vs_finite_differences(kernel):
+    X, y = _design(25, 2, 9)
+    theta = np.array([0.4, 0.6])
+    alpha = 0.9
+    _, g = OB.log_marg_post(X, y, kernel, theta, alpha=alpha)
+    fd = np.zeros(3)
+    for k in range(3):
+        tp, tm = theta.copy(), theta.copy()
+        ap = am = alpha
+        if k < 2:
+            h = 1e-6 * theta[k]
+            tp[k] += h
+            tm[k] -= h
+        else:
+            h = 1e-7
+            ap, am = alpha + h, alpha - h
+        fd[k] = (OB.log_marg_post(X, y, kernel, tp, alpha=ap, want_grad=False)[0]
+                 - OB.log_marg_post(X, y, kernel, tm, alpha=am, want_grad=False)[0]) / (2 * h)
+    np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-8 * np.max(np.abs(fd)))
