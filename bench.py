@@ -87,6 +87,17 @@ def rinv_traffic_bytes():
     return None
 
 
+def mfma_probe_sweep(N):
+    """FP64 MFMA issue rate from registers (lkgpu_mfma_probe) over independent
+    chains per wave and workgroups (256 threads) per launch"""
+    ctx = N.Context(0)
+    print(f"{'chains':>6s} {'workgroups':>10s} {'ms':>9s} {'TFLOP/s':>8s}")
+    for chains in (1, 2, 4, 8):
+        for blocks in (256, 512, 1024, 2048):
+            ms, tf = ctx.mfma_probe(blocks, 20000 if blocks <= 1024 else 10000, chains)
+            print(f"{chains:6d} {blocks:10d} {ms:9.3f} {tf:8.2f}", flush=True)
+
+
 def cholesky_large(N, n=8192, d=10, reps=3):
     """The single-evaluation Cholesky at n = 8192 (gauss kernel, d = 10,
     theta log-U[0.3, 1.5], nugget ratio 1e-6), timed live over its own
@@ -189,12 +200,17 @@ def main():
     ap.add_argument("--stage", default="adjoint", help="kernel timed live for the roofline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help=f"also run a {COPY_BYTES >> 20} MiB device copy (counter calibration)")
+    ap.add_argument("--mfma-probe-sweep", action="store_true",
+                    help="only print the FP64 MFMA issue-rate probe over chains x workgroups")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the multistart batch would shard by starts, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    if args.mfma_probe_sweep:
+        mfma_probe_sweep(N)
+        return
     if args.workload == "nested":
         run_nested(args)
         return
@@ -290,6 +306,10 @@ def main():
     }
     copy_ms = ctx.device_copy(COPY_BYTES, 5)
     line["hbm_copy_GBps"] = 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9  # read + write of a plain copy kernel
+    # FP64 MFMA rate reachable from registers alone (8 independent
+    # accumulation chains per wave, 8 waves per SIMD): the practical ceiling
+    # beside the published 78.6 TFLOP/s
+    line["mfma_f64_probe_TFLOPs"] = ctx.mfma_probe(2048, 10000, 8)[1]
     if not args.no_chol:
         line["cholesky_n8192"] = cholesky_large(N)
     if not args.no_cpu:

@@ -123,10 +123,12 @@ __device__ inline void pivot_sqrt(double a, double& root, double& rroot) {
 }
 
 struct DiagLds {
-  double P[2][NB * 4];  // current values of 4 columns, row-major [row][4]
-  double Q[2][NB * 4];  // their L values
+  // LDS layouts keep a wave's 64 lanes on consecutive doubles for every
+  // per-lane access (stride 1 writes / reads, or one broadcast address)
+  double P[2][4 * NB];  // current values of 4 columns, [column][row]
+  double Q[2][4 * NB];  // their L values, [column][row]
   double L[NB * (NB + 1)];  // L_kk column-major, stride NB + 1
-  double X[NB * NB];        // L_kk^-1 rows staged for coalesced stores
+  double X[NB * (NB + 1)];  // L_kk^-1 rows staged for coalesced stores, stride NB + 1
   double R[2][4 * NB];      // 4 rows of L_kk^-1 of the current inverse step
   double diag[NB], rinv[NB];
 };
@@ -142,7 +144,7 @@ __device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
     double* Q = sh.Q[p & 1];
     if (W == (p >> 2)) {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) P[l * 4 + s] = a[(j0 & 15) + s];
+      for (int s = 0; s < 4; ++s) P[s * NB + l] = a[(j0 & 15) + s];
     }
     __syncthreads();
     // 4 x 4 diagonal sub-block, factored by every lane
@@ -150,7 +152,7 @@ __device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int c = 0; c <= r; ++c) b[r][c] = P[(j0 + r) * 4 + c];
+      for (int c = 0; c <= r; ++c) b[r][c] = P[c * NB + j0 + r];
     double L4[4][4], iv[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -172,7 +174,7 @@ __device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
     {
       double v[4];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) v[s] = P[l * 4 + s];
+      for (int s = 0; s < 4; ++s) v[s] = P[s * NB + l];
 #pragma unroll
       for (int s = 0; s < 4; ++s) {
         double t = v[s];
@@ -191,11 +193,13 @@ __device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
         }
       }
     }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) Q[l * 4 + s] = x[s];
+    // every wave computed the same x; the owner of the columns publishes it
     if (W == (p >> 2)) {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) a[(j0 & 15) + s] = x[s];
+      for (int s = 0; s < 4; ++s) {
+        Q[s * NB + l] = x[s];
+        a[(j0 & 15) + s] = x[s];
+      }
     }
     if (W == 0 && l == 0) {
 #pragma unroll
@@ -212,7 +216,7 @@ __device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
       if (c >= j0 + 4) {
         double t = a[q];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) t = fma(-x[s], Q[c * 4 + s], t);
+        for (int s = 0; s < 4; ++s) t = fma(-x[s], Q[s * NB + c], t);
         a[q] = t;
       }
     }
@@ -239,7 +243,7 @@ __device__ inline void diag_inverse(DiagLds& sh, int l) {
         for (int m = 0; m < s; ++m) t = fma(-sh.L[(j0 + m) * (NB + 1) + j0 + s], x[m], t);
         x[s] = t * sh.rinv[j0 + s];
         Rr[s * NB + l] = x[s];
-        sh.X[(j0 + s) * NB + l] = x[s];
+        sh.X[(j0 + s) * (NB + 1) + l] = x[s];
       }
     }
     __syncthreads();
@@ -311,7 +315,7 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(const Slot* __restrict__ slo
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = 16 * w + q;
-    Wb[(size_t)c * ld + l] = sh.X[l * NB + c];
+    Wb[(size_t)c * ld + l] = sh.X[l * (NB + 1) + c];
   }
   (void)sbad;
 }
