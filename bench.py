@@ -209,6 +209,7 @@ def main():
         sys.exit(2)
 
     if args.mfma_probe_sweep:
+        from libkriging_amd import _native as N
         mfma_probe_sweep(N)
         return
     if args.workload == "nested":
