@@ -209,6 +209,16 @@ int lk_kriging_log_marg_post_fun(void* ptr, const double* theta, int theta_n, in
                                  double* grad_out);
 /* libkriging_c.h:134 (Kriging::logMargPost at the fitted parameters) */
 double lk_kriging_log_marg_post(void* ptr);
+/* libkriging_c.h:83-91 (Kriging::update, Kriging.cpp:2433-2633): n new
+ * points (X_u n x d column-major); noise_u must be empty (no heterogeneous
+ * noise on this path) */
+int lk_kriging_update(void* ptr, const double* y_u, int n, const double* noise_u, int noise_n, const double* X_u,
+                      int nX, int d, int refit);
+/* libkriging_c.h:105-106 (Kriging::save / Kriging::load, Kriging.cpp:
+ * 2691-2750): the reference's JSON version-2 model file; load returns a new
+ * handle (NULL on error) whose factors are recomputed on the device */
+int lk_kriging_save(void* ptr, const char* filename);
+void* lk_kriging_load(const char* filename);
 /* libkriging_c.h:133 (Kriging::leaveOneOut at the fitted theta) */
 double lk_kriging_leave_one_out(void* ptr);
 /* libkriging_c.h:136 (Kriging::leaveOneOutVec, Kriging.cpp:478-485): n

@@ -24,6 +24,8 @@ SOURCES = [
     "lbfgsb.cpp",
     "kriging.cpp",
     "kriging_capi.cpp",
+    "kriging_io.cpp",
+    "model_io.cpp",
     "nested.cpp",
     "nested_capi.cpp",
 ]

@@ -147,19 +147,25 @@ void Kriging::setup(const double* y, const double* X, int n, int d, RegModel reg
   for (int k = 0; k < d; ++k)
     for (int i = 0; i < n; ++i) X_[i + (size_t)k * n] = (X[i + (size_t)k * n] - centerX_[k]) / scaleX_[k];
   for (int i = 0; i < n; ++i) y_[i] = (y[i] - centerY_) / scaleY_;
-  // Trend::regressionModelMatrix (Trend.cpp:34-61)
-  p_ = regmodel == RegModel::None ? 0 : (regmodel == RegModel::Constant ? 1 : 1 + d);
+  upload_data();
+  is_empty_ = true;
+}
+
+// Trend::regressionModelMatrix (Trend.cpp:34-61) of the normalised design,
+// then the design, y and F uploaded as this model's device problem
+void Kriging::upload_data() {
+  const int n = n_, d = d_;
+  p_ = regmodel_ == RegModel::None ? 0 : (regmodel_ == RegModel::Constant ? 1 : 1 + d);
   F_.assign((size_t)n * std::max(p_, 1), 0.0);
   if (p_ > 0)
     for (int i = 0; i < n; ++i) F_[i] = 1.0;
-  if (regmodel == RegModel::Linear)
+  if (regmodel_ == RegModel::Linear)
     for (int k = 0; k < d; ++k)
       for (int i = 0; i < n; ++i) F_[i + (size_t)(k + 1) * n] = X_[i + (size_t)k * n];
-
   rt_ = &runtime_for(device_);
   if (problem_ >= 0) rt_->remove_problem(problem_);
+  problem_ = -1;
   problem_ = rt_->add_problem(X_.data(), n, d, y_.data(), F_.data(), p_, kernel_id(kernel_), (int)noise_);
-  is_empty_ = true;
 }
 
 ModelFlags Kriging::flags_for_eval() const {
@@ -379,6 +385,7 @@ struct FitState : FitJob {
   int d = 0, gd = 0;
   bool nug = false;
   int obj = OBJ_LL;  // LL and LMP are maximised (the optimiser sees -f), LOO minimised
+  bool no_restart = false;  // update's warm start: one optimiser run, no restart rules
   int n = 0;
   Reparam rp{0, false, true};
   double sign() const { return obj == OBJ_LOO ? 1.0 : -1.0; }
@@ -451,7 +458,7 @@ struct FitState : FitJob {
       rp.from_gamma(w.gamma.data(), theta_part.data(), &a_tmp);
       double sol_to_lb = std::numeric_limits<double>::infinity();
       for (int k = 0; k < d; ++k) sol_to_lb = std::min(sol_to_lb, std::fabs(theta_part[k] - th_lo[k]));
-      const bool restart = (w.retry < oc.max_restart) &&
+      const bool restart = !no_restart && (w.retry < oc.max_restart) &&
                            (w.opt->task().rfind("ABNORMAL_TERMINATION_IN_LNSRCH", 0) == 0 ||
                             w.opt->num_iters() <= 2 || sol_to_lb < std::numeric_limits<double>::epsilon() ||
                             fopt > w.best_f);
@@ -747,6 +754,103 @@ void Kriging::run_fit_jobs(const std::vector<FitJob*>& jobs_in) {
   std::vector<EvalResult> cres;
   rt->evaluate(creq, false, cres);
   for (size_t j = 0; j < jobs.size(); ++j) jobs[j]->m->commit_apply(jobs[j]->theta, jobs[j]->alpha, (int)j, cres[j]);
+}
+
+// ---------------------------------------------------------------------------
+// update (Kriging::update, Kriging.cpp:2433-2633): append m points
+// ---------------------------------------------------------------------------
+void Kriging::update(const double* y_u, const double* X_u, int m, bool refit) {
+  if (is_empty_) throw std::runtime_error("model is not fitted");
+  if (m < 0) throw std::invalid_argument("negative number of new points");
+  const int n = n_, d = d_, N = n + m;
+  if (refit && optim_ != "none" && noise_ == NoiseModel::Nugget) {
+    // nugget model: a full re-fit on the de-normalised data, from the
+    // current parameters unless every one is estimated (Kriging.cpp:2447-2467)
+    std::vector<double> y_all(N), X_all((size_t)N * d);
+    for (int i = 0; i < n; ++i) y_all[i] = y_[i] * scaleY_ + centerY_;
+    for (int i = 0; i < m; ++i) y_all[n + i] = y_u[i];
+    for (int k = 0; k < d; ++k) {
+      for (int i = 0; i < n; ++i) X_all[i + (size_t)k * N] = X_[i + (size_t)k * n] * scaleX_[k] + centerX_[k];
+      for (int i = 0; i < m; ++i) X_all[n + i + (size_t)k * N] = X_u[i + (size_t)k * m];
+    }
+    Parameters P;
+    if (!(est_beta_ && est_nugget_ && est_sigma2_ && est_theta_)) {
+      P.sigma2 = sigma2_ * scaleY_ * scaleY_;
+      P.is_sigma2_estim = est_sigma2_;
+      std::vector<double> th(d);
+      for (int k = 0; k < d; ++k) th[k] = theta_[k] * scaleX_[k];
+      P.theta = th;
+      P.theta_rows = 1;
+      P.is_theta_estim = est_theta_;
+      P.nugget = nugget_ * scaleY_ * scaleY_;
+      P.is_nugget_estim = est_nugget_;
+      if (!est_beta_) {
+        std::vector<double> b(beta_);
+        for (double& v : b) v *= scaleY_;
+        P.beta = b;
+        P.is_beta_estim = false;
+      }
+    }
+    const std::string optim = optim_, objective = objective_;
+    fit(y_all.data(), X_all.data(), N, d, regmodel_, normalize_, optim, objective, P);
+    return;
+  }
+  // the new points in the existing normalisation (KrigingImpl.cpp:580-596)
+  std::vector<double> Xn((size_t)N * d), yn(N);
+  for (int k = 0; k < d; ++k) {
+    for (int i = 0; i < n; ++i) Xn[i + (size_t)k * N] = X_[i + (size_t)k * n];
+    for (int i = 0; i < m; ++i) Xn[n + i + (size_t)k * N] = (X_u[i + (size_t)k * m] - centerX_[k]) / scaleX_[k];
+  }
+  for (int i = 0; i < n; ++i) yn[i] = y_[i];
+  for (int i = 0; i < m; ++i) yn[n + i] = (y_u[i] - centerY_) / scaleY_;
+  X_.swap(Xn);
+  y_.swap(yn);
+  n_ = N;
+  upload_data();
+  if (refit && optim_ != "none") {
+    // warm restart: one L-BFGS-B run from the current theta within the
+    // bounds of the extended design, no restarts (Kriging.cpp:2469-2633)
+    const OptimConfig& oc = optim_config();
+    auto st = std::make_unique<FitState>();
+    st->m = this;
+    st->d = d;
+    st->gd = d;
+    st->bfgs = true;
+    st->no_restart = true;
+    st->obj = objective_ == "LOO" ? OBJ_LOO : (objective_ == "LMP" ? OBJ_LMP : OBJ_LL);
+    st->n = N;
+    st->rp = Reparam{d, false, oc.reparametrize};
+    theta_bounds(st->th_lo, st->th_up);
+    std::vector<double> glo(d), gup(d);
+    st->rp.to_gamma(st->th_lo.data(), 1.0, glo.data());
+    st->rp.to_gamma(st->th_up.data(), 1.0, gup.data());
+    st->W.resize(1);
+    st->th_alpha.assign(1, std::vector<double>(d));
+    StartWorker& w = st->W[0];
+    w.theta_start = theta_;
+    w.glo = glo;
+    w.gup = gup;
+    st->restart_point(w, theta_);
+    w.best_gamma = w.gamma;
+    st->flags = flags_for_eval();
+    est_theta_ = true;
+    run_fit_jobs({st.get()});
+    return;
+  }
+  // no re-fit: the model at the current parameters on the extended design,
+  // sigma2 = SSE / n when estimated (KrigingImpl.cpp:576-632)
+  std::vector<double> g(theta_);
+  if (noise_ == NoiseModel::Nugget) g.push_back(alpha_);
+  std::vector<EvalRequest> reqs{commit_request(g)};
+  reqs[0].flags.alpha = alpha_;
+  std::vector<EvalResult> res;
+  rt_->evaluate(reqs, false, res);
+  const double s2 = sigma2_, nug = nugget_;
+  const bool est_theta = est_theta_;
+  commit_apply(theta_, alpha_, 0, res[0]);
+  est_theta_ = est_theta;
+  sigma2_ = est_sigma2_ ? res[0].sse / N : s2;
+  nugget_ = nug;
 }
 
 // ---------------------------------------------------------------------------

@@ -96,6 +96,15 @@ class Kriging {
   double leaveOneOutFun(const double* theta, int theta_n, bool want_grad, double* grad);
   void leaveOneOutVec(const double* theta, int theta_n, double* yhat, double* stderr_out);
   double leaveOneOut();
+  // Kriging::update (Kriging.cpp:2433-2633): append m points (X_u m x d
+  // column-major); refit = warm-started re-optimisation (a full re-fit for
+  // the nugget model), else the model at the current parameters
+  void update(const double* y_u, const double* X_u, int m, bool refit);
+  // model files in the reference's JSON version-2 format (Kriging::save /
+  // Kriging::load, Kriging.cpp:2691-2750); load recomputes the factors on
+  // the device at the saved parameters
+  void save(const std::string& filename);
+  static Kriging* load(const std::string& filename);
   // log marginal posterior (Kriging::logMargPostFun / logMargPost,
   // Kriging.cpp:488-672); gamma as logLikelihoodFun
   double logMargPostFun(const double* gamma, int gamma_n, bool want_grad, double* grad);
@@ -142,6 +151,7 @@ class Kriging {
  private:
   friend struct FitState;
   void setup(const double* y, const double* X, int n, int d, RegModel regmodel, bool normalize);
+  void upload_data();
   ModelFlags flags_for_eval() const;
   // commit in two halves so that many models commit in one batch: the
   // request, then the copy of slot `slot`'s factors once it is evaluated

@@ -207,6 +207,38 @@ int lk_kriging_leave_one_out_vec(void* ptr, const double* theta, int theta_n, do
   });
 }
 
+int lk_kriging_update(void* ptr, const double* y_u, int n, const double* noise_u, int noise_n, const double* X_u,
+                      int nX, int d, int refit) {
+  return guarded([&] {
+    (void)noise_u;
+    if (noise_n > 0) throw std::invalid_argument("heterogeneous noise is not supported on this path");
+    lk::Kriging* k = K(ptr);
+    if (n != nX)
+      throw std::runtime_error("Dimension of new data should be the same:\n X: (" + std::to_string(nX) + "x" +
+                               std::to_string(d) + "), y: (" + std::to_string(n) + ")");
+    if (d != k->d())
+      throw std::runtime_error("Dimension of new data should be the same:\n X: (...x" + std::to_string(k->d()) +
+                               "), new X: (...x" + std::to_string(d) + ")");
+    k->update(y_u, X_u, n, refit != 0);
+  });
+}
+
+int lk_kriging_save(void* ptr, const char* filename) {
+  return guarded([&] {
+    if (filename == nullptr) throw std::invalid_argument("filename is NULL");
+    K(ptr)->save(filename);
+  });
+}
+
+void* lk_kriging_load(const char* filename) {
+  void* out = nullptr;
+  guarded([&] {
+    if (filename == nullptr) throw std::invalid_argument("filename is NULL");
+    out = lk::Kriging::load(filename);
+  });
+  return out;
+}
+
 double lk_kriging_log_likelihood(void* ptr) {
   double v = 0.0;
   if (guarded([&] { v = K(ptr)->logLikelihood(); }) != 0) return std::numeric_limits<double>::quiet_NaN();

@@ -186,6 +186,8 @@ hipError_t launch_loo_bw(const Batch& b);
 hipError_t launch_loo_bs(const Batch& b);
 hipError_t launch_loo_adjoint(const Batch& b);
 hipError_t launch_loo_finalize_grad(const Batch& b);
+// R^-1 = L^-T L^-1 in full into A (after an L^-1-path evaluation)
+hipError_t launch_rinv_store(const Batch& b);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
 // chains in {1, 2, 4, 8}; out: blocks * 256 doubles
 hipError_t launch_mfma_probe(int blocks, int iters, int chains, double* out, hipStream_t stream);

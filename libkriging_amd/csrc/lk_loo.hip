@@ -382,8 +382,38 @@ __global__ __launch_bounds__(TPB) void k_loo_finalize_grad(const Slot* __restric
 }
 
 // ---------------------------------------------------------------------------
+// R^-1 = L^-T L^-1 stored in full into A (the model file's "Rinv").  Lower
+// tiles on MFMA, each stored with its mirror.  grid (tiles, nslots)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TPB) void k_rinv_store(const Slot* __restrict__ slots,
+                                                    const SlotParams* __restrict__ params,
+                                                    SlotOut* __restrict__ out) {
+  const TileSlot ts = tile_slot();
+  const int b = ts.slot;
+  const Slot S = slots[b];
+  if (!slot_live(params[b], out[b])) return;
+  if (ts.tile >= tri_n(S.nblk)) return;
+  int it, jt;
+  tri_decode(ts.tile, it, jt);
+  const int ld = S.npad;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  const int i0 = it * NB, j0 = jt * NB;
+  __shared__ double sbuf[4 * KC * LDSS];
+  d4 acc[2][2];
+  acc_zero(acc);
+  tile_mainloop<true, true, false>(S.W, ld, i0, S.W, ld, j0, it * NB, S.nblk * NB, sbuf, acc, tid, lane, wr, wc);
+  acc_store(acc, S.A + (size_t)j0 * ld + i0, ld, lane, wr, wc);
+  if (it != jt) acc_store_t(acc, S.A + (size_t)i0 * ld + j0, ld, lane, wr, wc);
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+hipError_t launch_rinv_store(const Batch& B) {
+  hipLaunchKernelGGL(k_rinv_store, dim3(tri_n(B.nblk_max), B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params,
+                     B.out);
+  return hipGetLastError();
+}
 hipError_t launch_loo_g(const Batch& B) {
   hipLaunchKernelGGL(k_loo_g, dim3(B.nblk_max, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   return hipGetLastError();

@@ -580,6 +580,26 @@ Runtime::SlotView Runtime::slot_view(int s) const {
   return SlotView{S.A, S.W, S.V, S.n, S.npad, S.p};
 }
 
+void Runtime::read_rinv(int s, double* out) {
+  if (last_rinv_) throw std::runtime_error("R^-1 needs an evaluation on the L^-1 path");
+  if (s < 0 || s >= nslots_) throw std::invalid_argument("slot out of range");
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  Batch B;
+  B.slots = d_slots_;
+  B.params = d_params_;
+  B.out = d_out_;
+  B.nslots = nslots_;
+  B.nblk_max = nblk_max_;
+  B.dmax = dmax_;
+  B.stream = stream_;
+  hip_check(launch_rinv_store(B), "R^-1");
+  const SlotView v = slot_view(s);
+  hip_check(hipStreamSynchronize(stream_), "sync");
+  hip_check(hipMemcpy2D(out, sizeof(double) * v.n, v.A, sizeof(double) * v.npad, sizeof(double) * v.n, v.n,
+                        hipMemcpyDeviceToHost),
+            "read R^-1");
+}
+
 void Runtime::read_slot(int s, int what, double* out) {
   const SlotView v = slot_view(s);
   if (last_rinv_ && what >= 1 && what <= 4)

@@ -109,6 +109,9 @@ class Runtime {
   // host copy of a slot buffer after the last evaluate(); `what` as
   // LKGPU_READ_* in include/lkgpu.h
   void read_slot(int s, int what, double* out);
+  // R^-1 (n x n, full, column-major) of slot s after the last evaluate() of
+  // the L^-1 path; overwrites the slot's L
+  void read_rinv(int s, double* out);
 
   CholConfig chol;
   bool use_graph = true;

@@ -37,6 +37,10 @@ N.register("lk_kriging_leave_one_out_fun", ctypes.c_int, [_vp, _dp, ctypes.c_int
 N.register("lk_kriging_leave_one_out", ctypes.c_double, [_vp])
 N.register("lk_kriging_log_marg_post_fun", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp])
 N.register("lk_kriging_log_marg_post", ctypes.c_double, [_vp])
+N.register("lk_kriging_update", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int])
+N.register("lk_kriging_save", ctypes.c_int, [_vp, _cp])
+N.register("lk_kriging_load", _vp, [_cp])
 N.register("lk_kriging_leave_one_out_vec", ctypes.c_int, [_vp, _dp, ctypes.c_int, _dp, _dp])
 for _name in ("lk_kriging_kernel", "lk_kriging_optim", "lk_kriging_objective", "lk_kriging_regmodel",
               "lk_kriging_noise_model"):
@@ -202,6 +206,31 @@ class Kriging:
                                                             int(bool(return_grad)), N.ptr(ll),
                                                             grad.ctypes.data_as(_dp) if grad is not None else None))
         return ll, grad
+
+    def update(self, y_u, X_u, refit=True, noise_u=None):
+        """Kriging::update (Kriging.cpp:2433-2633): append the points (X_u, y_u)"""
+        if noise_u is not None:
+            raise ValueError("heterogeneous noise is not supported on this path")
+        y = _col(y_u)
+        X = _mat(X_u)
+        n, d = X.shape
+        N.check(N.lib().lk_kriging_update(self._h, N.ptr(y), y.size, None, 0, N.ptr(X), n, d, int(bool(refit))))
+
+    def save(self, filename):
+        """the reference's JSON version-2 model file (Kriging::save)"""
+        N.check(N.lib().lk_kriging_save(self._h, _b(str(filename))))
+
+    @classmethod
+    def load(cls, filename):
+        """a model from a JSON version-2 file (Kriging::load)"""
+        h = N.lib().lk_kriging_load(_b(str(filename)))
+        if not h:
+            raise RuntimeError(N.lib().lk_get_last_error().decode())
+        k = cls.__new__(cls)
+        k._h = h
+        k._owned = True
+        k._keep = None
+        return k
 
     def leaveOneOutFun(self, theta, return_grad=False):
         """(loo, grad) as Kriging::leaveOneOutFun (Kriging.cpp:470-476)"""

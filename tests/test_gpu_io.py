@@ -1,0 +1,168 @@
+"""Model files (Kriging::save / load, Kriging.cpp:2691-2750; the reference's
+JSON version-2 format) and Kriging::update (Kriging.cpp:2433-2633)."""
+import base64
+import json
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# every field KrigingImpl::load_common_from_json and Kriging::load read
+REF_KEYS = ["version", "content", "covType", "X", "centerX", "scaleX", "y", "centerY", "scaleY", "normalize",
+            "regmodel", "optim", "objective", "dX", "maxdX", "F", "T", "R", "M", "star", "circ", "z", "Rinv", "beta",
+            "est_beta", "theta", "est_theta", "sigma2", "est_sigma2", "noise_model"]
+
+
+def _arr(node):
+    raw = np.frombuffer(base64.b64decode(node["base64_data"]), dtype="<f8")
+    if node["type"] == "mat":
+        return raw.reshape((node["n_cols"], node["n_rows"])).T
+    assert raw.size == node["size"]
+    return raw.copy()
+
+
+def _design(n, d, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(size=(n, d)) * 2.0 + 1.0
+    y = 5.0 + np.sin(3 * X[:, 0]) + X[:, -1] ** 2 + 0.01 * rng.normal(size=n)
+    return X, y
+
+
+def test_save_has_the_reference_fields(tmp_path):
+    import libkriging_amd as lk
+    X, y = _design(90, 2, 1)
+    k = lk.Kriging(y, X, "matern5_2", "linear", True, "BFGS", "LL")
+    f = tmp_path / "k.json"
+    k.save(f)
+    j = json.loads(f.read_text())
+    for key in REF_KEYS:
+        assert key in j, key
+    assert j["version"] == 2 and j["content"] == "Kriging" and j["covType"] == "matern5_2"
+    assert j["regmodel"] == "linear" and j["normalize"] is True and j["noise_model"] == "None"
+    Xs, ys = _arr(j["X"]), _arr(j["y"])
+    assert Xs.shape == (90, 2) and ys.shape == (90,)
+    np.testing.assert_allclose(Xs * _arr(j["scaleX"]) + _arr(j["centerX"]), X, rtol=1e-14)
+    np.testing.assert_allclose(ys * j["scaleY"] + j["centerY"], y, rtol=1e-14)
+    T, R, Rinv = _arr(j["T"]), _arr(j["R"]), _arr(j["Rinv"])
+    c = np.linalg.cond(R)
+    assert np.allclose(np.triu(T, 1), 0.0)
+    assert np.max(np.abs(T @ T.T - R)) < 1e-12
+    assert np.max(np.abs(Rinv @ R - np.eye(90))) < 1e-13 * c
+    th = _arr(j["theta"])
+    np.testing.assert_allclose(th, k.theta().reshape(-1), rtol=0)
+    Rk = O.corr_mat("matern5_2", Xs, Xs, th)
+    np.fill_diagonal(Rk, 1.0)
+    np.testing.assert_allclose(R, Rk, rtol=1e-13, atol=1e-15)
+    F, M = _arr(j["F"]), _arr(j["M"])
+    assert F.shape == (90, 3) and M.shape == (90, 3)
+    np.testing.assert_allclose(M, np.linalg.solve(T, F), rtol=1e-9, atol=1e-11)
+    dX = _arr(j["dX"])
+    assert dX.shape == (2, 90 * 90)
+    np.testing.assert_allclose(dX[:, 3 * 90 + 7], Xs[3] - Xs[7], rtol=0, atol=0)
+    np.testing.assert_allclose(dX[:, 7 * 90 + 3], Xs[3] - Xs[7], rtol=0, atol=0)
+    np.testing.assert_allclose(_arr(j["maxdX"]), np.max(np.abs(Xs[:, None, :] - Xs[None, :, :]), axis=(0, 1)),
+                               rtol=0)
+    assert _arr(j["star"]).size == 0 and _arr(j["circ"]).shape == (3, 3)
+    assert _arr(j["beta"]).shape == (3,)
+
+
+@pytest.mark.parametrize("noise", [None, "nugget"])
+def test_save_load_roundtrip(tmp_path, noise):
+    import libkriging_amd as lk
+    X, y = _design(120, 3, 2)
+    k = lk.Kriging(y, X, "gauss", "constant", False, "BFGS2", "LL", {}, noise)
+    f = tmp_path / "k.json"
+    k.save(f)
+    k2 = lk.Kriging.load(f)
+    Xn = np.random.default_rng(3).uniform(size=(50, 3)) * 2.0 + 1.0
+    m1, s1, _, _, _ = k.predict(Xn, True)
+    m2, s2, _, _, _ = k2.predict(Xn, True)
+    np.testing.assert_allclose(m2, m1, rtol=1e-12, atol=1e-12 * np.max(np.abs(m1)))
+    np.testing.assert_allclose(s2, s1, rtol=1e-10, atol=1e-12 * np.max(s1))
+    np.testing.assert_array_equal(k2.theta(), k.theta())
+    assert k2.sigma2() == k.sigma2() and k2.kernel() == "gauss" and k2.objective() == "LL"
+    assert rel_err(k2.logLikelihood(), k.logLikelihood()) < 1e-13
+    if noise:
+        assert k2.nugget() == k.nugget()
+
+
+def test_load_a_file_written_elsewhere(tmp_path):
+    """a file laid out as another writer would (keys in another order, other
+    whitespace, fields this path does not need): load rebuilds the model"""
+    import libkriging_amd as lk
+    X, y = _design(60, 2, 5)
+    ref = lk.Kriging(y, X, "exp", "constant", False, "none", "LL", {"theta": np.array([[0.4, 0.7]])})
+
+    def node_mat(a):
+        a = np.asarray(a, dtype="<f8")
+        return {"type": "mat", "n_rows": a.shape[0], "n_cols": a.shape[1],
+                "base64_data": base64.b64encode(np.asfortranarray(a).T.tobytes()).decode()}
+
+    def node_vec(a, t):
+        a = np.asarray(a, dtype="<f8").reshape(-1)
+        return {"type": t, "size": a.size, "base64_data": base64.b64encode(a.tobytes()).decode()}
+
+    doc = {"content": "Kriging", "version": 2, "covType": "exp", "X": node_mat(X), "y": node_vec(y, "colvec"),
+           "centerX": node_vec([0.0, 0.0], "rowvec"), "scaleX": node_vec([1.0, 1.0], "rowvec"),
+           "centerY": 0.0, "scaleY": 1.0, "normalize": False, "regmodel": "constant", "optim": "none",
+           "objective": "LL", "theta": node_vec([0.4, 0.7], "colvec"), "est_theta": False,
+           "beta": node_vec(ref.beta(), "colvec"), "est_beta": True, "sigma2": ref.sigma2(), "est_sigma2": True,
+           "dX": node_mat(np.zeros((2, 0))), "T": node_mat(np.zeros((0, 0))), "nystrom_light": False}
+    f = tmp_path / "ext.json"
+    f.write_text(json.dumps(doc, indent=2))
+    k = lk.Kriging.load(f)
+    Xn = np.random.default_rng(6).uniform(size=(30, 2))
+    np.testing.assert_allclose(k.predict(Xn, True)[0], ref.predict(Xn, True)[0], rtol=1e-12, atol=1e-13)
+    with pytest.raises(RuntimeError, match="requires 2"):
+        f2 = tmp_path / "v1.json"
+        doc["version"] = 1
+        f2.write_text(json.dumps(doc))
+        lk.Kriging.load(f2)
+
+
+def test_update_without_refit_equals_model_on_all_data():
+    import libkriging_amd as lk
+    X, y = _design(100, 2, 7)
+    th = np.array([[0.6, 0.9]])
+    k = lk.Kriging(y[:80], X[:80], "matern3_2", "linear", False, "none", "LL", {"theta": th})
+    k.update(y[80:], X[80:], False)
+    full = lk.Kriging(y, X, "matern3_2", "linear", False, "none", "LL", {"theta": th})
+    Xn = np.random.default_rng(8).uniform(size=(40, 2)) * 2.0 + 1.0
+    np.testing.assert_allclose(k.predict(Xn, True)[0], full.predict(Xn, True)[0], rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(k.predict(Xn, True)[1], full.predict(Xn, True)[1], rtol=1e-9, atol=1e-12)
+    assert rel_err(k.sigma2(), full.sigma2()) < 1e-12
+    assert k.X().shape == (100, 2)
+
+
+def test_update_refit_warm_start_improves_the_likelihood():
+    import libkriging_amd as lk
+    X, y = _design(130, 2, 9)
+    k = lk.Kriging(y[:100], X[:100], "gauss", "constant", True, "BFGS", "LL")
+    old = k.theta().reshape(-1).copy()
+    k.update(y[100:], X[100:], True)
+    new = k.theta().reshape(-1)
+    # on the extended data the warm-started optimum is no worse than the old theta
+    assert k.logLikelihoodFun(new)[0] >= k.logLikelihoodFun(old)[0] - 1e-9
+    assert k.X().shape == (130, 2)
+
+
+def test_update_refit_nugget_is_a_full_refit():
+    import libkriging_amd as lk
+    X, y = _design(90, 2, 11)
+    k = lk.Kriging(y[:70], X[:70], "matern5_2", "constant", False, "BFGS", "LL", {}, "nugget")
+    k.update(y[70:], X[70:], True)
+    full = lk.Kriging(y, X, "matern5_2", "constant", False, "BFGS", "LL", {}, "nugget")
+    np.testing.assert_allclose(k.theta(), full.theta(), rtol=1e-10)
+    assert rel_err(k.nugget(), full.nugget()) < 1e-9
+
+
+def test_update_rejects_a_wrong_dimension():
+    import libkriging_amd as lk
+    X, y = _design(40, 2, 13)
+    k = lk.Kriging(y, X, "gauss", "constant", False, "none", "LL", {"theta": np.array([[0.5, 0.5]])})
+    with pytest.raises(RuntimeError, match="Dimension of new data"):
+        k.update(np.ones(3), np.ones((3, 3)), False)
