@@ -13,7 +13,10 @@ namespace lk {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));  // loads as one 16-byte vector
 
-constexpr int KC = 16;          // K-chunk staged in LDS per step
+#ifndef LK_KC
+#define LK_KC 16
+#endif
+constexpr int KC = LK_KC;       // K-chunk staged in LDS per step
 #ifndef LK_PREFETCH_DEPTH
 #define LK_PREFETCH_DEPTH 3
 #endif
