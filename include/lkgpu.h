@@ -128,6 +128,14 @@ int lkgpu_synchronize(lkgpu_context* ctx);
 /* Use (1) or bypass (0) hipGraph replay of the evaluation pipeline. */
 int lkgpu_set_graph(lkgpu_context* ctx, int enable);
 
+/* Most streams (1, 2 or 4; default 4, or LKGPU_STREAMS) an evaluation splits
+ * its batch over (no reference counterpart): a batch of B slots runs as S
+ * equal parts on S streams, S the largest allowed value with B a multiple of
+ * 8 S, so that latency-bound launches of one part overlap the matrix-core
+ * launches of the others.  Results do not depend on S (each slot's launches
+ * are the same).  Runs with a timed stage use one stream. */
+int lkgpu_set_streams(lkgpu_context* ctx, int streams);
+
 /* How gradient evaluations form R^-1 (no reference counterpart; both give the
  * reference's LL, gradient and condition-check decision):
  *   0 (default) L^-1 by recursive doubling, then R^-1 = L^-T L^-1 in one launch

@@ -31,6 +31,7 @@ SIGNATURES = {
     "lkgpu_context_destroy": (None, [_vp]),
     "lkgpu_set_chol_config": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
     "lkgpu_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lkgpu_set_streams": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lkgpu_problem_create": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, _ip]),
     "lkgpu_problem_destroy": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -117,6 +118,10 @@ class Context:
 
     def set_graph(self, enable):
         check(lib().lkgpu_set_graph(self.h, int(enable)))
+
+    def set_streams(self, streams):
+        """most streams (1, 2, 4) a batch is split over"""
+        check(lib().lkgpu_set_streams(self.h, int(streams)))
 
     def problem(self, X, y, F, kernel, noise=None):
         X = f64(X)

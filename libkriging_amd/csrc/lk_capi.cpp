@@ -131,6 +131,13 @@ int lkgpu_set_gradient_path(lkgpu_context* ctx, int path) {
   });
 }
 
+int lkgpu_set_streams(lkgpu_context* ctx, int streams) {
+  return guarded([&] {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    ctx->rt.set_streams(streams);
+  });
+}
+
 int lkgpu_gradient_path(lkgpu_context* ctx) {
   int v = -1;
   guarded([&] {

@@ -108,28 +108,35 @@ __global__ __launch_bounds__(TPB) void k_build(const Slot* __restrict__ slots, c
 // LinearAlgebra.cpp:71).
 // ---------------------------------------------------------------------------
 __device__ inline void pivot_sqrt(double a, double& root, double& rroot) {
-  double y = __builtin_amdgcn_rsq(a);
-  const double h = 0.5 * a;
-#pragma unroll
-  for (int it = 0; it < 2; ++it) {
-    const double r = fma(-h * y, y, 0.5);
-    y = fma(y, r, y);
-  }
-  double s = a * y;
-  const double e = fma(-s, s, a);
-  s = fma(e, 0.5 * y, s);
-  root = s;
-  rroot = y;
+  // y = v_rsq_f64(a) (relative error below 2^-22), then one third-order
+  // step: with e = 1 - a y^2, a^-1/2 = y (1 - e)^-1/2 = y (1 + e/2 + 3e^2/8
+  // + O(e^3)); two dependent operations fewer than two Newton steps
+  const double y = __builtin_amdgcn_rsq(a);
+  const double t = a * y;
+  const double e = fma(-t, y, 1.0);
+  const double p = fma(e, 0.375, 0.5);
+  const double iv = fma(y * e, p, y);
+  double s = a * iv;
+  const double r = fma(-s, s, a);
+  root = fma(r, 0.5 * iv, s);
+  rroot = iv;
 }
+
+// columns (factor) / rows (inverse) per step of the diagonal-block sweeps
+#ifndef LK_DIAG_CW
+#define LK_DIAG_CW 4
+#endif
+constexpr int DCW = LK_DIAG_CW;
+static_assert(DCW == 4 || DCW == 8, "diagonal sweep step must be 4 or 8 columns");
 
 struct DiagLds {
   // LDS layouts keep a wave's 64 lanes on consecutive doubles for every
   // per-lane access (stride 1 writes / reads, or one broadcast address)
-  double P[2][4 * NB];  // current values of 4 columns, [column][row]
-  double Q[2][4 * NB];  // their L values, [column][row]
-  double L[NB * (NB + 1)];  // L_kk column-major, stride NB + 1
+  double P[2][DCW * NB];  // current values of the step's columns, [column][row]
+  double Q[2][DCW * NB];  // their L values, [column][row]
+  double L[NB * (NB + 2)];  // L_kk column-major, stride NB + 2 (16-byte aligned column pairs)
   double X[NB * (NB + 1)];  // L_kk^-1 rows staged for coalesced stores, stride NB + 1
-  double R[2][4 * NB];      // 4 rows of L_kk^-1 of the current inverse step
+  double R[2][DCW * NB];    // the step's rows of L_kk^-1 in the inverse sweep
   double diag[NB], rinv[NB];
 };
 
@@ -138,85 +145,84 @@ template <int W>
 __device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
   bool ok = true;
 #pragma unroll
-  for (int p = 0; p < 16; ++p) {
-    const int j0 = 4 * p;
+  for (int p = 0; p < NB / DCW; ++p) {
+    const int j0 = DCW * p;
     double* P = sh.P[p & 1];
     double* Q = sh.Q[p & 1];
-    if (W == (p >> 2)) {
+    if (W == j0 / 16) {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) P[s * NB + l] = a[(j0 & 15) + s];
+      for (int s = 0; s < DCW; ++s) P[s * NB + l] = a[(j0 & 15) + s];
     }
     __syncthreads();
-    // 4 x 4 diagonal sub-block, factored by every lane
-    double b[4][4];
+    // DCW x DCW diagonal sub-block, factored by every lane (left-looking)
+    double L4[DCW][DCW], iv[DCW];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < DCW; ++r)
 #pragma unroll
-      for (int c = 0; c <= r; ++c) b[r][c] = P[c * NB + j0 + r];
-    double L4[4][4], iv[4];
+      for (int c = 0; c <= r; ++c) L4[r][c] = P[c * NB + j0 + r];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      double d = b[c][c];
+    for (int c = 0; c < DCW; ++c) {
+      double d = L4[c][c];
 #pragma unroll
       for (int m = 0; m < c; ++m) d = fma(-L4[c][m], L4[c][m], d);
       ok = ok && (d > 0.0);
       pivot_sqrt(d, L4[c][c], iv[c]);
 #pragma unroll
-      for (int r = c + 1; r < 4; ++r) {
-        double t = b[r][c];
+      for (int r = c + 1; r < DCW; ++r) {
+        double t = L4[r][c];
 #pragma unroll
         for (int m = 0; m < c; ++m) t = fma(-L4[r][m], L4[c][m], t);
         L4[r][c] = t * iv[c];
       }
     }
-    // this row's L values in columns j0 .. j0+3
-    double x[4];
+    // this row's L values in columns j0 .. j0+DCW-1
+    double x[DCW];
     {
-      double v[4];
+      double v[DCW];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) v[s] = P[s * NB + l];
+      for (int s = 0; s < DCW; ++s) v[s] = P[s * NB + l];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+      for (int s = 0; s < DCW; ++s) {
         double t = v[s];
 #pragma unroll
         for (int m = 0; m < s; ++m) t = fma(-x[m], L4[s][m], t);
         x[s] = t * iv[s];
       }
-      if (l < j0 + 4) {
+      // rows inside the block reproduce L4 below the diagonal (same
+      // operations); the diagonal takes the corrected root, rows above the
+      // block are zero.  Entries above the diagonal are left as computed:
+      // they only reach the strict upper part of this tile's columns, which
+      // no lower-part value reads and the store zeroes.
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-          double vv = 0.0;
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            if (l == j0 + r && s <= r) vv = L4[r][s];
-          x[s] = (l < j0) ? 0.0 : vv;
-        }
+      for (int s = 0; s < DCW; ++s) {
+        x[s] = (l == j0 + s) ? L4[s][s] : x[s];
+        x[s] = (l < j0) ? 0.0 : x[s];
       }
     }
     // every wave computed the same x; the owner of the columns publishes it
-    if (W == (p >> 2)) {
+    if (W == j0 / 16) {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+      for (int s = 0; s < DCW; ++s) {
         Q[s * NB + l] = x[s];
         a[(j0 & 15) + s] = x[s];
       }
     }
     if (W == 0 && l == 0) {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+      for (int s = 0; s < DCW; ++s) {
         sh.diag[j0 + s] = L4[s][s];
         sh.rinv[j0 + s] = iv[s];
       }
     }
     __syncthreads();
-    // rank-4 update of this wave's columns right of the block
+    // rank-DCW update of this wave's columns right of the block
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int c = 16 * W + q;
-      if (c >= j0 + 4) {
+      if (c >= j0 + DCW) {
         double t = a[q];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) t = fma(-x[s], Q[s * NB + c], t);
+        for (int s = 0; s < DCW; ++s) t = fma(-x[s], Q[s * NB + c], t);
         a[q] = t;
       }
     }
@@ -231,16 +237,16 @@ __device__ inline void diag_inverse(DiagLds& sh, int l) {
 #pragma unroll
   for (int q = 0; q < 16; ++q) acc[q] = 0.0;
 #pragma unroll
-  for (int p = 0; p < 16; ++p) {
-    const int j0 = 4 * p;
+  for (int p = 0; p < NB / DCW; ++p) {
+    const int j0 = DCW * p;
     double* Rr = sh.R[p & 1];
-    if (W == (p >> 2)) {
-      double x[4];
+    if (W == j0 / 16) {
+      double x[DCW];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+      for (int s = 0; s < DCW; ++s) {
         double t = ((l == j0 + s) ? 1.0 : 0.0) - acc[(j0 & 15) + s];
 #pragma unroll
-        for (int m = 0; m < s; ++m) t = fma(-sh.L[(j0 + m) * (NB + 1) + j0 + s], x[m], t);
+        for (int m = 0; m < s; ++m) t = fma(-sh.L[(j0 + m) * (NB + 2) + j0 + s], x[m], t);
         x[s] = t * sh.rinv[j0 + s];
         Rr[s * NB + l] = x[s];
         sh.X[(j0 + s) * (NB + 1) + l] = x[s];
@@ -250,10 +256,10 @@ __device__ inline void diag_inverse(DiagLds& sh, int l) {
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int i = 16 * W + q;
-      if (i >= j0 + 4) {
+      if (i >= j0 + DCW) {
         double t = acc[q];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) t = fma(sh.L[(j0 + s) * (NB + 1) + i], Rr[s * NB + l], t);
+        for (int s = 0; s < DCW; ++s) t = fma(sh.L[(j0 + s) * (NB + 2) + i], Rr[s * NB + l], t);
         acc[q] = t;
       }
     }
@@ -297,7 +303,7 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(const Slot* __restrict__ slo
     const int c = 16 * w + q;
     const double v = (l >= c) ? a[q] : 0.0;
     Ab[(size_t)c * ld + l] = v;
-    sh.L[c * (NB + 1) + l] = v;
+    sh.L[c * (NB + 2) + l] = v;
   }
   __syncthreads();
   if (w == 0) {

@@ -22,6 +22,24 @@ Runtime::Runtime(int device) : device_(device) {
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   if (const char* e = std::getenv("LKGPU_PANEL_BLOCKS")) panel_blocks_ = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("LKGPU_GRAD_PATH")) rinv_path_ = std::atoi(e) != 0;
+  int S = 4;
+  if (const char* e = std::getenv("LKGPU_STREAMS")) {
+    const int v = std::atoi(e);
+    S = (v == 2 || v == 4) ? v : 1;
+  }
+  set_streams(S);
+}
+
+void Runtime::set_streams(int S) {
+  if (S != 1 && S != 2 && S != 4) throw std::invalid_argument("streams must be 1, 2 or 4");
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  if (S > 1 && fork_ == nullptr) hip_check(hipEventCreateWithFlags(&fork_, hipEventDisableTiming), "hipEventCreate");
+  for (int q = 0; q + 1 < S; ++q) {
+    if (xstream_[q] == nullptr)
+      hip_check(hipStreamCreateWithFlags(&xstream_[q], hipStreamNonBlocking), "hipStreamCreate");
+    if (join_[q] == nullptr) hip_check(hipEventCreateWithFlags(&join_[q], hipEventDisableTiming), "hipEventCreate");
+  }
+  streams_ = S;
 }
 
 Runtime::~Runtime() {
@@ -41,6 +59,11 @@ Runtime::~Runtime() {
   if (d_out_) hipFree(d_out_);
   if (h_params_) hipHostFree(h_params_);
   if (h_out_) hipHostFree(h_out_);
+  if (fork_) hipEventDestroy(fork_);
+  for (int q = 0; q < 3; ++q) {
+    if (join_[q]) hipEventDestroy(join_[q]);
+    if (xstream_[q]) hipStreamDestroy(xstream_[q]);
+  }
   if (stream_) hipStreamDestroy(stream_);
 }
 
@@ -307,6 +330,33 @@ void Runtime::take_stage_time(double* avg_ms, int* count) {
   timed_count_ = 0;
 }
 
+// Parts the batch is split into: the largest S <= streams_ (4, 2) whose
+// parts keep the XCD-aware placement (slots a multiple of 8 S); 1 for timed
+// runs, whose stage events sit on the main stream.
+int Runtime::split_parts(bool timed, BenchMap* bench) const {
+  if (timed || bench != nullptr) return 1;
+  for (int S = streams_; S > 1; S /= 2)
+    if (nslots_ % (8 * S) == 0) return S;
+  return 1;
+}
+
+// launches of the S parts, interleaved, between a fork from and a join back
+// to the main stream (also valid inside a stream capture, which the other
+// streams then join)
+void Runtime::issue_split(const std::vector<Step>* parts, int S) {
+  hip_check(hipEventRecord(fork_, stream_), "fork record");
+  for (int q = 1; q < S; ++q) hip_check(hipStreamWaitEvent(xstream_[q - 1], fork_, 0), "fork wait");
+  size_t len = 0;
+  for (int q = 0; q < S; ++q) len = std::max(len, parts[q].size());
+  for (size_t i = 0; i < len; ++i)
+    for (int q = 0; q < S; ++q)
+      if (i < parts[q].size()) hip_check(parts[q][i].fn(), parts[q][i].name);
+  for (int q = 1; q < S; ++q) {
+    hip_check(hipEventRecord(join_[q - 1], xstream_[q - 1]), "join record");
+    hip_check(hipStreamWaitEvent(stream_, join_[q - 1], 0), "join wait");
+  }
+}
+
 void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
   Batch B;
   B.slots = d_slots_;
@@ -321,6 +371,21 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
   const std::vector<Step> steps = pipeline_steps(B, want_grad, objective);
   const auto rng = stage_range(steps, timed_stage_);
   const bool timed = rng.first < rng.second;
+  const int S = split_parts(timed, bench);
+  const bool split = S > 1;
+  std::vector<Step> part[4];
+  if (split) {
+    const int h = nslots_ / S;
+    for (int q = 0; q < S; ++q) {
+      Batch H = B;
+      H.nslots = h;
+      H.slots = B.slots + q * h;
+      H.params = B.params + q * h;
+      H.out = B.out + q * h;
+      H.stream = q == 0 ? stream_ : xstream_[q - 1];
+      part[q] = pipeline_steps(H, want_grad, objective);
+    }
+  }
   if (bench != nullptr) {
     // uncaptured run with one event pair around the whole pipeline
     hipEvent_t e0, e1;
@@ -338,11 +403,21 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
   } else if (use_graph) {
     if (objective < 0 || objective >= OBJ_COUNT) throw std::invalid_argument("unknown objective");
     GraphEntry& G = graphs_[2 * objective + (want_grad ? 1 : 0)];
-    if (G.version != layout_version_ || G.exec == nullptr || G.timed_stage != timed_stage_) {
+    if (G.version != layout_version_ || G.exec == nullptr || G.timed_stage != timed_stage_ || G.split != S) {
       if (G.exec) hip_check(hipGraphExecDestroy(G.exec), "graph exec destroy");
       G.exec = nullptr;
       hipGraph_t graph = nullptr;
-      if (!timed) {
+      if (split) {
+        hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+          issue_split(part, S);
+        } catch (...) {
+          hipStreamEndCapture(stream_, &graph);
+          if (graph) hipGraphDestroy(graph);
+          throw;
+        }
+        hip_check(hipStreamEndCapture(stream_, &graph), "end capture");
+      } else if (!timed) {
         graph = capture(steps, 0, steps.size());
       } else {
         // events captured from a stream are not timeable here, so the timed
@@ -373,8 +448,11 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
       hip_check(hipGraphDestroy(graph), "graph destroy");
       G.version = layout_version_;
       G.timed_stage = timed_stage_;
+      G.split = S;
     }
     hip_check(hipGraphLaunch(G.exec, stream_), "graph launch");
+  } else if (split) {
+    issue_split(part, S);
   } else {
     run_steps(steps, 0, rng.first);
     if (timed) hip_check(hipEventRecord(tev_[0], stream_), "timing event");

@@ -118,6 +118,11 @@ class Runtime {
   void set_gradient_path(int path) { rinv_path_ = path != 0; }
   void recheck_rcond_bound(int ns, BenchMap* bench);
   int gradient_path() const { return rinv_path_ ? 1 : 0; }
+  // most parts (1, 2 or 4) an untimed evaluation splits its batch into, one
+  // stream each; a batch is split into the largest S <= this with slots a
+  // multiple of 8 S.  Also set by LKGPU_STREAMS.
+  void set_streams(int S);
+  int streams() const { return streams_; }
 
   // Live timing of one pipeline stage with a HIP event pair recorded on the
   // pipeline stream (inside the captured graph when graphs are used).
@@ -167,6 +172,7 @@ class Runtime {
   struct GraphEntry {
     uint64_t version = ~0ull;
     int timed_stage = -1;
+    int split = 1;
     hipGraphExec_t exec = nullptr;
   };
   GraphEntry graphs_[2 * OBJ_COUNT];
@@ -180,6 +186,15 @@ class Runtime {
   // kind of the last run on the slots: L^-1, F*, y*, E* are formed only by
   // evaluations that do not take the R^-1 sweep
   bool last_rinv_ = false;
+  // the batch split into S equal parts on S streams: the latency-bound
+  // launches of one part (diagonal blocks, panels, finalisation) run beside
+  // the MFMA launches of the others
+  int streams_ = 4;
+  hipStream_t xstream_[3] = {nullptr, nullptr, nullptr};  // streams of parts 1 .. S-1
+  hipEvent_t fork_ = nullptr;
+  hipEvent_t join_[3] = {nullptr, nullptr, nullptr};
+  int split_parts(bool timed, BenchMap* bench) const;
+  void issue_split(const std::vector<Step>* parts, int S);
   hipEvent_t tev_[2] = {nullptr, nullptr};
   double timed_sum_ms_ = 0.0;
   int timed_count_ = 0;

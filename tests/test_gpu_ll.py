@@ -193,6 +193,43 @@ def test_graph_replay_matches_eager(ctx):
         ctx.drop_problem(pid)
 
 
+def test_stream_split_matches_one_stream(ctx):
+    # a batch split over 4 or 2 streams (4 or 2 parts, each on its own graph
+    # branch) equals the one-stream eager result bit for bit, on every replay,
+    # with eager runs in between; uniform and uneven batches
+    pids = []
+    try:
+        X, y = _design(41, 200, 3, "matern5_2")
+        pids.append(ctx.problem(X, y, np.ones((200, 1)), "matern5_2"))
+        uneven = []
+        for i, n in enumerate([70, 300, 129, 64, 250, 33, 190, 96] * 4):
+            X, y = _design(200 + i, n, 3, "matern5_2")
+            uneven.append(ctx.problem(X, y, np.ones((n, 1)), "matern5_2"))
+        pids += uneven
+        rng = np.random.default_rng(2)
+        for ids in ([pids[0]] * 64, uneven):
+            gam = np.exp(rng.uniform(-1.5, 0.5, size=(len(ids), 3)))
+            ctx.set_streams(1)
+            ctx.set_graph(False)
+            ll0, g0, st0, inc0 = ctx.ll_grad_batch(ids, gam, True)
+            assert (st0 == 0).all()
+            for S in (4, 2):
+                ctx.set_streams(S)
+                for rep in range(3):
+                    ctx.set_graph(True)
+                    ll1, g1, st1, inc1 = ctx.ll_grad_batch(ids, gam, True)
+                    assert np.array_equal(ll1, ll0) and np.array_equal(g1, g0)
+                    assert np.array_equal(st1, st0) and np.array_equal(inc1, inc0)
+                    ctx.set_graph(False)
+                    ll2, g2, _, _ = ctx.ll_grad_batch(ids, gam, True)
+                    assert np.array_equal(ll2, ll0) and np.array_equal(g2, g0)
+    finally:
+        ctx.set_streams(4)
+        ctx.set_graph(True)
+        for p in pids:
+            ctx.drop_problem(p)
+
+
 def test_batch_equals_single(ctx):
     # a batched evaluation must equal the same evaluations one per launch
     X, y = _design(31, 400, 6, "exp")
