@@ -315,6 +315,21 @@ __device__ inline double kern_rho(int kern, const double* dx, const double* it) 
   }
 }
 
+// g_k = d ln rho / d theta_k for one coordinate (Covariance.cpp:38-179),
+// it = 1 / theta_k: the same expressions as kern_rho_g, one k at a time
+__device__ inline double kern_g(int kern, double dx, double it) {
+  if (kern == K_GAUSS) return (dx * dx) * ((it * it) * it);
+  if (kern == K_EXP) return fabs(dx) * (it * it);
+  if (kern == K_MATERN32) {
+    const double u = SQRT3 * fabs(dx * it);
+    return (u * u) / (1.0 + u) * it;
+  }
+  const double u = SQRT5 * fabs(dx * it);
+  const double aa = 1.0 + u;
+  const double bb = (u * u) * (1.0 / 3.0);
+  return (aa * bb) / (aa + bb) * it;
+}
+
 // rho(dx) and g_k = d ln rho / d theta_k (Covariance.cpp:38-179); it = 1 / theta,
 // it2 = it^2, entries k >= d hold dx = 0 (their g is 0)
 template <int DT>

@@ -687,7 +687,8 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
 // R^-1 = Linv^T Linv is accumulated on MFMA; then every strictly-lower
 // element (gi > gj) contributes, with dR_ij/dtheta_k = R_ij g_k(x_j - x_i)
 // recomputed from X (KrigingImpl.cpp:871-884, Kriging.cpp:303-316):
-//   s1_k += x_i x_j R_ij g_k      s2_k += Rinv_ij R_ij g_k
+//   s1_k += (x_i x_j - sigma2 Rinv_ij) R_ij g_k   (the finaliser's
+//           s1 / sigma2 - s2 in one sum; the s2 partials are zero)
 //   a1   += x_i x_j rho_ij        a2   += Rinv_ij rho_ij
 // and diagonal elements add x_i^2 and Rinv_ii.  Partials per tile go to
 // red[grad_off + tile * GRAD_STRIDE]: [s1 (DT) | s2 (DT) | a1 a2 dg1 dg2].
@@ -748,21 +749,18 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
   }
   __syncthreads();
 
-  double it_[DT], it2[DT];
+  double it_[DT];
 #pragma unroll
-  for (int k = 0; k < DT; ++k) {
-    const double t = (k < d) ? P.theta[k] : 1.0;
-    it_[k] = 1.0 / t;
-    it2[k] = it_[k] * it_[k];
-  }
+  for (int k = 0; k < DT; ++k) it_[k] = 1.0 / ((k < d) ? P.theta[k] : 1.0);
   const double factor = (S.noise == NOISE_NUGGET) ? P.alpha : 1.0;
   const int kern = S.kernel;
-  double s1[DT], s2[DT];
+  // one sum per theta_k: sum (x_i x_j - sigma2 Rinv_ij) dR_ij/dtheta_k, the
+  // finaliser's s1 / sigma2 - s2 with sigma2 (k_finalize_ll, earlier in the
+  // pipeline) folded in per element; the s2 partials are written as zero
+  const double s2g = out[b].sigma2;
+  double sg[DT];
 #pragma unroll
-  for (int k = 0; k < DT; ++k) {
-    s1[k] = 0.0;
-    s2[k] = 0.0;
-  }
+  for (int k = 0; k < DT; ++k) sg[k] = 0.0;
   double a1 = 0.0, a2 = 0.0, dg1 = 0.0, dg2 = 0.0;
 
 #pragma unroll
@@ -784,56 +782,15 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
           continue;
         }
         double dx[DT];
-        double s = 0.0, pr = 1.0;
 #pragma unroll
-        for (int k = 0; k < DT; ++k) {
-          dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
-        }
-        double g[DT];
-        double rho;
-        if (kern == K_GAUSS) {
-#pragma unroll
-          for (int k = 0; k < DT; ++k) {
-            const double v = dx[k] * it_[k];
-            s = fma(v, v, s);
-            g[k] = (dx[k] * dx[k]) * (it2[k] * it_[k]);
-          }
-          rho = exp(-0.5 * s);
-        } else if (kern == K_EXP) {
-#pragma unroll
-          for (int k = 0; k < DT; ++k) {
-            s += fabs(dx[k] * it_[k]);
-            g[k] = fabs(dx[k]) * it2[k];
-          }
-          rho = exp(-s);
-        } else if (kern == K_MATERN32) {
-#pragma unroll
-          for (int k = 0; k < DT; ++k) {
-            const double u = SQRT3 * fabs(dx[k] * it_[k]);
-            s += u;
-            pr *= 1.0 + u;
-            g[k] = (u * u) / (1.0 + u) * it_[k];
-          }
-          rho = exp(-s) * pr;
-        } else {
-#pragma unroll
-          for (int k = 0; k < DT; ++k) {
-            const double u = SQRT5 * fabs(dx[k] * it_[k]);
-            const double aa = 1.0 + u;
-            const double bb = (u * u) * (1.0 / 3.0);
-            s += u;
-            pr *= aa + bb;
-            g[k] = (aa * bb) / (aa + bb) * it_[k];
-          }
-          rho = exp(-s) * pr;
-        }
+        for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
+        // rho first, then each g_k recomputed from dx_k while accumulating:
+        // no g[] held across the exp (fewer live registers, more waves)
+        const double rho = kern_rho<DT>(kern, dx, it_);
         const double Rij = rho * factor;
+        const double wgt = fma(-s2g, rinv, xx);
 #pragma unroll
-        for (int k = 0; k < DT; ++k) {
-          const double gk = Rij * g[k];
-          s1[k] = fma(xx, gk, s1[k]);
-          s2[k] = fma(rinv, gk, s2[k]);
-        }
+        for (int k = 0; k < DT; ++k) sg[k] = fma(wgt, Rij * kern_g(kern, dx[k], it_[k]), sg[k]);
         a1 = fma(xx, rho, a1);
         a2 = fma(rinv, rho, a2);
       }
@@ -843,11 +800,10 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
   double* dst = S.red + red_grad_off(S.nblk) + (size_t)ts.tile * GRAD_STRIDE;
 #pragma unroll
   for (int k = 0; k < DT; ++k) {
-    const double v1 = wave_sum(s1[k]);
-    const double v2 = wave_sum(s2[k]);
+    const double v1 = wave_sum(sg[k]);
     if (lane == 0) {
       sred[w][k] = v1;
-      sred[w][DT + k] = v2;
+      sred[w][DT + k] = 0.0;
     }
   }
   {
@@ -863,8 +819,8 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
   if (tid < 2 * DT + 4) {
     const double v = sred[0][tid] + sred[1][tid] + sred[2][tid] + sred[3][tid];
     int q;
-    if (tid < DT) q = tid;                       // s1
-    else if (tid < 2 * DT) q = DMAX + tid - DT;  // s2
+    if (tid < DT) q = tid;                       // s1 (sigma2 folded in, see above)
+    else if (tid < 2 * DT) q = DMAX + tid - DT;  // s2 (zero)
     else q = 2 * DMAX + (tid - 2 * DT);          // a1 a2 dg1 dg2
     dst[q] = v;
   }
