@@ -240,6 +240,15 @@ __device__ inline void diag_inverse(DiagLds& sh, int l) {
   for (int p = 0; p < NB / DCW; ++p) {
     const int j0 = DCW * p;
     double* Rr = sh.R[p & 1];
+    // this wave's L(i, j0 .. j0+DCW-1) for the update below, read before the
+    // barrier (L is read-only here): the loads overlap the owner's chain
+    // instead of following the barrier
+    double Lu[16][DCW];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (16 * W + q >= j0 + DCW)
+#pragma unroll
+        for (int s = 0; s < DCW; ++s) Lu[q][s] = sh.L[(j0 + s) * (NB + 2) + 16 * W + q];
     if (W == j0 / 16) {
       double x[DCW];
 #pragma unroll
@@ -259,7 +268,7 @@ __device__ inline void diag_inverse(DiagLds& sh, int l) {
       if (i >= j0 + DCW) {
         double t = acc[q];
 #pragma unroll
-        for (int s = 0; s < DCW; ++s) t = fma(sh.L[(j0 + s) * (NB + 2) + i], Rr[s * NB + l], t);
+        for (int s = 0; s < DCW; ++s) t = fma(Lu[q][s], Rr[s * NB + l], t);
         acc[q] = t;
       }
     }
