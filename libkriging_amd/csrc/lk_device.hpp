@@ -20,11 +20,21 @@ constexpr int KC = LK_KC;       // K-chunk staged in LDS per step
 #ifndef LK_PREFETCH_DEPTH
 #define LK_PREFETCH_DEPTH 3
 #endif
-// LDS row stride (doubles) of a KC x NB staging tile.  66: the transposed
-// staging (16 threads writing 16 different k rows of one column pair) then
-// falls on 16 distinct even banks (2 * 2 * 66 = 264 = 8 mod 32 words), no
-// conflict; a stride of 68 (= 16 mod 32) put them on 2 banks (8-way).
+// LDS row strides (doubles) of a KC x NB staging tile.  LDS serves 16 lanes
+// of a 64-bit access per pass over 32 four-byte banks (the measured bank
+// conflicts follow that model).  A transposed operand is staged by 16 lanes
+// writing 8 k rows 2 apart and 2 columns: row steps of 4 S words must be
+// distinct mod 32 for the 8 rows, i.e. S odd: S = NB + 1 (NB + 2 put rows k
+// and k + 8 on one bank pair, 2-way; NB + 4, 4-way).  A plain operand is
+// staged as 16-byte column pairs, conflict-free with an even stride that
+// keeps them 16-byte aligned: S = NB + 2 (NB + 1 splits them, 2-way).  The
+// MFMA operand reads are conflict-free with either.  LDSS is the pitch the
+// buffers are sized with.
 constexpr int LDSS = NB + 2;
+template <bool TRANS>
+struct LdsStride {
+  static constexpr int v = TRANS ? NB + 1 : NB + 2;
+};
 constexpr int TPB = 256;        // threads of the tile kernels (4 waves)
 
 // ---------------------------------------------------------------------------
@@ -103,14 +113,14 @@ __device__ inline bool slot_live(const SlotParams& P, const SlotOut& O) {
 // op(M)(i0 + ii, k0 + kk):
 //   TRANS = false: op(M)(i, k) = M[k * ld + i]   (i contiguous in memory)
 //   TRANS = true : op(M)(i, k) = M[i * ld + k]   (k contiguous in memory)
-template <bool NEG, int KCH = KC>
+template <bool NEG, int KCH = KC, int SA = LDSS, int SB = LDSS>
 __device__ inline void mma_chunk(const double* __restrict__ sA, const double* __restrict__ sB, d4 (&acc)[2][2],
                                  int lane, int wr, int wc) {
   const int r = lane & 15, kq = lane >> 4;
 #pragma unroll
   for (int k0 = 0; k0 < KCH; k0 += 4) {
-    const double* pa = sA + (k0 + kq) * LDSS + wr * 32 + r;
-    const double* pb = sB + (k0 + kq) * LDSS + wc * 32 + r;
+    const double* pa = sA + (k0 + kq) * SA + wr * 32 + r;
+    const double* pb = sB + (k0 + kq) * SB + wc * 32 + r;
     double a0 = pa[0], a1 = pa[16];
     const double b0 = pb[0], b1 = pb[16];
     if (NEG) {
@@ -155,6 +165,7 @@ __device__ inline void chunk_load(ChunkRegs& r, const double* __restrict__ M, in
 
 template <bool TRANS>
 __device__ inline void chunk_store(const ChunkRegs& r, double* __restrict__ dst, int tid) {
+  constexpr int LDSS = LdsStride<TRANS>::v;
   if (!TRANS) {
     const int ii = 2 * (tid & 31);
 #pragma unroll
@@ -250,7 +261,7 @@ __device__ inline void tile_mainloop(const double* __restrict__ A, int lda, int 
         chunk_load<TA>(ra[u % D], A, lda, ai0, kbeg + (t + D) * KC, tid);
         chunk_load<TB>(rb[u % D], B, ldb, bj0, kbeg + (t + D) * KC, tid);
       }
-      mma_chunk<NEG>(sA[lb], sB[lb], acc, lane, wr, wc);
+      mma_chunk<NEG, KC, LdsStride<TA>::v, LdsStride<TB>::v>(sA[lb], sB[lb], acc, lane, wr, wc);
     }
   }
   // the caller may reuse sbuf: wait until every wave has finished reading it

@@ -240,8 +240,9 @@ __global__ __launch_bounds__(TPB) void k_loo_bs(const Slot* __restrict__ slots, 
   const int p = S.p, ld = S.npad;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const int i0 = it * NB, j0 = jt * NB;
-  __shared__ double sbuf[4 * KC * LDSS];
-  static_assert(4 * KC * LDSS >= 2 * PMAX * NB, "G rows of both blocks fit the staging buffer");
+  // staging chunks, then the G rows of both blocks (2 PMAX NB doubles)
+  constexpr int kBuf = 4 * KC * LDSS > 2 * PMAX * NB ? 4 * KC * LDSS : 2 * PMAX * NB;
+  __shared__ double sbuf[kBuf];
   __shared__ double sai[NB], saj[NB];
   d4 acc[2][2];
   acc_zero(acc);

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(TPB, 2) void k_nk_pair(NkArgs a) {
 #pragma unroll
       for (int r = 0; r < WPT; ++r) {
         const int q = tid + r * TPB;
-        sB[(q % PKC) * LDSS + q / PKC] = rw[r];
+        sB[(q % PKC) * (NB + 1) + q / PKC] = rw[r];  // odd stride: 16 lanes on 16 bank pairs
       }
 #pragma unroll
       for (int r = 0; r < (DMAX * PKC) / TPB; ++r) {
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(TPB, 2) void k_nk_pair(NkArgs a) {
         sA[kk * LDSS + ii] = kern_rho<DT>(a.kernel, dx, it);
       }
       __syncthreads();
-      mma_chunk<false, PKC>(sA, sB, acc, lane, wr, wc);
+      mma_chunk<false, PKC, LDSS, NB + 1>(sA, sB, acc, lane, wr, wc);
       __syncthreads();
     }
     // epilogue: weight by W_i(a, t) and sum this tile's rows into part
