@@ -60,7 +60,7 @@ Runtime::~Runtime() {
   if (h_params_) hipHostFree(h_params_);
   if (h_out_) hipHostFree(h_out_);
   if (fork_) hipEventDestroy(fork_);
-  for (int q = 0; q < 3; ++q) {
+  for (int q = 0; q < kMaxStreams - 1; ++q) {
     if (join_[q]) hipEventDestroy(join_[q]);
     if (xstream_[q]) hipStreamDestroy(xstream_[q]);
   }
@@ -373,7 +373,7 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
   const bool timed = rng.first < rng.second;
   const int S = split_parts(timed, bench);
   const bool split = S > 1;
-  std::vector<Step> part[4];
+  std::vector<Step> part[kMaxStreams];
   if (split) {
     const int h = nslots_ / S;
     for (int q = 0; q < S; ++q) {

@@ -190,9 +190,10 @@ class Runtime {
   // launches of one part (diagonal blocks, panels, finalisation) run beside
   // the MFMA launches of the others
   int streams_ = 4;
-  hipStream_t xstream_[3] = {nullptr, nullptr, nullptr};  // streams of parts 1 .. S-1
+  static constexpr int kMaxStreams = 4;
+  hipStream_t xstream_[kMaxStreams - 1] = {};  // streams of parts 1 .. S-1
   hipEvent_t fork_ = nullptr;
-  hipEvent_t join_[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t join_[kMaxStreams - 1] = {};
   int split_parts(bool timed, BenchMap* bench) const;
   void issue_split(const std::vector<Step>* parts, int S);
   hipEvent_t tev_[2] = {nullptr, nullptr};
