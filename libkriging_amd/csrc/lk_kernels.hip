@@ -101,8 +101,10 @@ __global__ __launch_bounds__(TPB) void k_build(const Slot* __restrict__ slots, c
 // owns columns (factor) or rows (inverse) 16w .. 16w+15 in registers.  Both
 // sweeps go 4 columns / rows at a time: a 4 x 4 diagonal sub-block is
 // factored redundantly by every lane from LDS broadcasts, then a rank-4
-// update; 16 sequential steps per sweep instead of 64.  Pivot square roots
-// come from v_rsq_f64 refined by two Newton steps.  Writes L_kk to A and
+// update; 16 sequential steps per sweep instead of 64, one workgroup barrier
+// per step (every wave computes the L values of all 64 rows and reads its
+// columns' values back from its own LDS copy).  Pivot square roots come from
+// v_rsq_f64 refined by one third-order step.  Writes L_kk to A and
 // L_kk^-1 to the same block of W (both with zero strict upper part), the
 // partial log-det, and flags a non-positive / NaN pivot (dpotrf failure,
 // LinearAlgebra.cpp:71).
@@ -133,7 +135,7 @@ struct DiagLds {
   // LDS layouts keep a wave's 64 lanes on consecutive doubles for every
   // per-lane access (stride 1 writes / reads, or one broadcast address)
   double P[2][DCW * NB];  // current values of the step's columns, [column][row]
-  double Q[2][DCW * NB];  // their L values, [column][row]
+  double Q[4][DCW * NB];  // their L values, [column][row], one copy per wave
   double L[NB * (NB + 2)];  // L_kk column-major, stride NB + 2 (16-byte aligned column pairs)
   double X[NB * (NB + 1)];  // L_kk^-1 rows staged for coalesced stores, stride NB + 1
   double R[2][DCW * NB];    // the step's rows of L_kk^-1 in the inverse sweep
@@ -148,7 +150,7 @@ __device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
   for (int p = 0; p < NB / DCW; ++p) {
     const int j0 = DCW * p;
     double* P = sh.P[p & 1];
-    double* Q = sh.Q[p & 1];
+    double* Q = sh.Q[W];
     if (W == j0 / 16) {
 #pragma unroll
       for (int s = 0; s < DCW; ++s) P[s * NB + l] = a[(j0 & 15) + s];
@@ -199,14 +201,16 @@ __device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
         x[s] = (l < j0) ? 0.0 : x[s];
       }
     }
-    // every wave computed the same x; the owner of the columns publishes it
+    // every wave computed the same x for all 64 rows: each keeps its own copy
+    // and reads its columns' values back without a workgroup barrier (one
+    // wave's LDS operations complete in order; the fences keep the compiler
+    // from moving the reads above the writes)
     if (W == j0 / 16) {
 #pragma unroll
-      for (int s = 0; s < DCW; ++s) {
-        Q[s * NB + l] = x[s];
-        a[(j0 & 15) + s] = x[s];
-      }
+      for (int s = 0; s < DCW; ++s) a[(j0 & 15) + s] = x[s];
     }
+#pragma unroll
+    for (int s = 0; s < DCW; ++s) Q[s * NB + l] = x[s];
     if (W == 0 && l == 0) {
 #pragma unroll
       for (int s = 0; s < DCW; ++s) {
@@ -214,7 +218,9 @@ __device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
         sh.rinv[j0 + s] = iv[s];
       }
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // rank-DCW update of this wave's columns right of the block
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
