@@ -391,6 +391,7 @@ __device__ inline double kern_rho_g(int kern, const double* dx, const double* it
 constexpr int MV_TPB = 512;
 constexpr int MV_W = MV_TPB / 64;
 constexpr int MV_U = 8;
+static_assert(4 * MV_U <= 64, "one lane per (input column, batch column)");
 
 // out[:, c] = Linv * in[:, c] for rows of block rb (grid (nblk_max, nslots))
 // in columns come from `src` (n rows, ld = n, zero beyond n) ; out to V.
@@ -417,20 +418,24 @@ __device__ inline void lower_matvec_block(const Slot& S, int rb, const double* _
       colp[c] = (src2 != nullptr && cs == ncols - 1) ? src2 : src + (size_t)cs * lds;
     }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    // lane c MV_U + u (< 4 MV_U) fetches input column c at k = kb + u: one
+    // coalesced load per batch instead of 4 MV_U same-address loads, each of
+    // which would move 64 lanes' worth of data; v_readlane broadcasts
+    const int cl = lane / MV_U, ul = lane % MV_U;
+    const double* my_col = colp[cl & 3];
     for (int kb = w * MV_U; kb < kend; kb += MV_W * MV_U) {
-      double lv[MV_U], sv[MV_U][4];
+      double lv[MV_U];
 #pragma unroll
       for (int u = 0; u < MV_U; ++u) {
         const int k = min(kb + u, kend - 1);
         lv[u] = gload(Wr + (size_t)k * ld);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) sv[u][c] = gload(colp[c] + k);
       }
+      const double sv = (lane < 4 * MV_U) ? gload(my_col + min(kb + ul, kend - 1)) : 0.0;
 #pragma unroll
       for (int u = 0; u < MV_U; ++u)
         if (kb + u < kend) {
 #pragma unroll
-          for (int c = 0; c < 4; ++c) acc[c] = fma(lv[u], sv[u][c], acc[c]);
+          for (int c = 0; c < 4; ++c) acc[c] = fma(lv[u], readlane_d(sv, c * MV_U + u), acc[c]);
         }
     }
 #pragma unroll
