@@ -189,10 +189,18 @@ __device__ inline void chunk_store(const ChunkRegs& r, double* __restrict__ dst,
 // acc += op(A)(ai0 + 0..63, k) op(B)(bj0 + 0..63, k)^T summed over
 // k in [kbeg, kend) (a multiple of KC), with op as above.  sbuf holds two
 // (A, B) chunk buffers: 4 * KC * LDSS doubles.  One barrier per chunk.
-template <bool TA, bool TB, bool NEG>
+// With SKIP, a wave with mma_on false stages chunks but issues no MFMA (its
+// quadrant of the tile is not needed; its acc is left as it was); without
+// it mma_on is ignored and the loop carries no such branch.
+template <bool TA, bool TB, bool NEG, bool SKIP = false>
 __device__ inline void tile_mainloop(const double* __restrict__ A, int lda, int ai0, const double* __restrict__ B,
                                      int ldb, int bj0, int kbeg, int kend, double* __restrict__ sbuf,
-                                     d4 (&acc)[2][2], int tid, int lane, int wr, int wc);
+                                     d4 (&acc)[2][2], int tid, int lane, int wr, int wc, bool mma_on = true);
+
+// the wave of a diagonal tile whose quadrant lies strictly above the
+// diagonal (rows 0..31, columns 32..63): symmetric and triangular products
+// need only the lower part of such tiles
+__device__ inline bool upper_quadrant(bool diagonal_tile, int wr, int wc) { return diagonal_tile && wr == 0 && wc == 1; }
 
 __device__ inline void acc_zero(d4 (&acc)[2][2]) {
 #pragma unroll
@@ -226,10 +234,10 @@ __device__ inline void acc_store(const d4 (&acc)[2][2], double* __restrict__ C, 
         C[(size_t)acc_col(lane, wc, bj, e) * ld + acc_row(lane, wr, bi)] = acc[bi][bj][e];
 }
 
-template <bool TA, bool TB, bool NEG>
+template <bool TA, bool TB, bool NEG, bool SKIP>
 __device__ inline void tile_mainloop(const double* __restrict__ A, int lda, int ai0, const double* __restrict__ B,
                                      int ldb, int bj0, int kbeg, int kend, double* __restrict__ sbuf,
-                                     d4 (&acc)[2][2], int tid, int lane, int wr, int wc) {
+                                     d4 (&acc)[2][2], int tid, int lane, int wr, int wc, bool mma_on) {
   if (kbeg >= kend) return;
   double* sA[2] = {sbuf, sbuf + 2 * KC * LDSS};
   double* sB[2] = {sbuf + KC * LDSS, sbuf + 3 * KC * LDSS};
@@ -261,7 +269,7 @@ __device__ inline void tile_mainloop(const double* __restrict__ A, int lda, int 
         chunk_load<TA>(ra[u % D], A, lda, ai0, kbeg + (t + D) * KC, tid);
         chunk_load<TB>(rb[u % D], B, ldb, bj0, kbeg + (t + D) * KC, tid);
       }
-      mma_chunk<NEG, KC, LdsStride<TA>::v, LdsStride<TB>::v>(sA[lb], sB[lb], acc, lane, wr, wc);
+      if (!SKIP || mma_on) mma_chunk<NEG, KC, LdsStride<TA>::v, LdsStride<TB>::v>(sA[lb], sB[lb], acc, lane, wr, wc);
     }
   }
   // the caller may reuse sbuf: wait until every wave has finished reading it

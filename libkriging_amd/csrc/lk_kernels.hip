@@ -733,7 +733,9 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
   d4 acc[2][2];
   acc_zero(acc);
   // Linv(k, i0 + ii) Linv(k, j0 + jj) over k >= it * NB
-  tile_mainloop<true, true, false>(S.W, ld, i0, S.W, ld, j0, it * NB, S.nblk * NB, sbuf, acc, tid, lane, wr, wc);
+  // (the epilogue reads only elements on or below the diagonal)
+  tile_mainloop<true, true, false, true>(S.W, ld, i0, S.W, ld, j0, it * NB, S.nblk * NB, sbuf, acc, tid, lane, wr,
+                                         wc, !upper_quadrant(it == jt, wr, wc));
 
   // epilogue: X rows of both blocks and the x vector in LDS (reuse sbuf)
   // the X rows need 2 DT NB doubles: in the staging buffer when it is large
