@@ -109,7 +109,7 @@ __device__ inline bool slot_live(const SlotParams& P, const SlotOut& O) {
 // column-major loads and stores of the C tile.
 // ---------------------------------------------------------------------------
 
-// Operand chunks are KC x NB slices of op(M) staged as dst[kk * LDSS + ii] =
+// Operand chunks are KC x NB slices of op(M) staged as dst[kk * S + ii] =
 // op(M)(i0 + ii, k0 + kk):
 //   TRANS = false: op(M)(i, k) = M[k * ld + i]   (i contiguous in memory)
 //   TRANS = true : op(M)(i, k) = M[i * ld + k]   (k contiguous in memory)
@@ -165,14 +165,14 @@ __device__ inline void chunk_load(ChunkRegs& r, const double* __restrict__ M, in
 
 template <bool TRANS>
 __device__ inline void chunk_store(const ChunkRegs& r, double* __restrict__ dst, int tid) {
-  constexpr int LDSS = LdsStride<TRANS>::v;
+  constexpr int S = LdsStride<TRANS>::v;  // row stride of this operand's staging
   if (!TRANS) {
     const int ii = 2 * (tid & 31);
 #pragma unroll
     for (int q = 0; q < CHUNK_V2; ++q) {
       const int kk = (tid >> 5) + 8 * q;
-      dst[kk * LDSS + ii] = r.v[q].x;
-      dst[kk * LDSS + ii + 1] = r.v[q].y;
+      dst[kk * S + ii] = r.v[q].x;
+      dst[kk * S + ii + 1] = r.v[q].y;
     }
   } else {
     constexpr int H = KC / 2;
@@ -180,8 +180,8 @@ __device__ inline void chunk_store(const ChunkRegs& r, double* __restrict__ dst,
 #pragma unroll
     for (int q = 0; q < CHUNK_V2; ++q) {
       const int ii = tid / H + (TPB / H) * q;
-      dst[kk * LDSS + ii] = r.v[q].x;
-      dst[(kk + 1) * LDSS + ii] = r.v[q].y;
+      dst[kk * S + ii] = r.v[q].x;
+      dst[(kk + 1) * S + ii] = r.v[q].y;
     }
   }
 }
