@@ -1196,11 +1196,15 @@ hipError_t launch_mfma_probe(int blocks, int iters, int chains, double* out, hip
 // (10) predictive covariance: Sigma = R_nn - C' C + Ecirc Ecirc' (before
 // var_scale), C = L^-1 R_on.  R_nn as LinearAlgebra::covMat_sym_X
 // (LinearAlgebra.cpp:531-596): factor * k(xn_i - xn_j) off the diagonal, 1
-// on it.  grid (mpad / 64, mpad / 64), full matrix
+// on it.  Sigma is symmetric: grid (T (T + 1) / 2), T = mpad / 64, over the
+// lower tiles, each off-diagonal tile stored at both of its places
 // ---------------------------------------------------------------------------
 template <int DT>
 __global__ __launch_bounds__(TPB) void k_predict_cov(PredictArgs a) {
-  const int j0 = blockIdx.x * NB, i0 = blockIdx.y * NB;
+  // lower tiles only; an off-diagonal tile also writes its transpose
+  int it, jt;
+  tri_decode(blockIdx.x, it, jt);
+  const int i0 = it * NB, j0 = jt * NB;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const int m = a.m, d = a.d;
   __shared__ double sbuf[4 * KC * LDSS];
@@ -1236,7 +1240,9 @@ __global__ __launch_bounds__(TPB) void k_predict_cov(PredictArgs a) {
         }
         double ec = 0.0;
         for (int q = 0; q < a.p; ++q) ec = fma(a.ecirc[(size_t)q * m + gi], a.ecirc[(size_t)q * m + gj], ec);
-        a.cov[(size_t)gj * m + gi] = acc[bi][bj][e] + rnn + ec;
+        const double v = acc[bi][bj][e] + rnn + ec;
+        a.cov[(size_t)gj * m + gi] = v;
+        if (it != jt) a.cov[(size_t)gi * m + gj] = v;
       }
 }
 
@@ -1346,7 +1352,8 @@ __global__ __launch_bounds__(TPB) void k_predict_deriv(PredictArgs a) {
 }
 
 hipError_t launch_predict_cov(const PredictArgs& a, hipStream_t stream) {
-  const dim3 grid(a.mpad / NB, a.mpad / NB);
+  const int T = a.mpad / NB;
+  const dim3 grid(T * (T + 1) / 2);
   if (a.d <= 4)
     hipLaunchKernelGGL(k_predict_cov<4>, grid, dim3(TPB), 0, stream, a);
   else if (a.d <= 8)
