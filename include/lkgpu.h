@@ -33,6 +33,11 @@ extern "C" {
  * (replaces libkriging_c.h:10 lk_get_last_error). */
 const char* lk_get_last_error(void);
 
+/* Hash of the sources this library was built from (no counterpart in the
+ * reference): the Python loader refuses a library whose hash differs from
+ * the sources beside it. */
+const char* lkgpu_build_id(void);
+
 /* ------------------------------------------------------------------------ */
 /* Layer 1: batched likelihood on one GPU                                   */
 /* ------------------------------------------------------------------------ */

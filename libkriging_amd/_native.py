@@ -8,6 +8,8 @@ import os
 
 import numpy as np
 
+from ._srchash import source_hash
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblkgpu.so")
 
@@ -27,6 +29,7 @@ SIGNATURES = {
     "lkgpu_mfma_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double)]),
     "lk_get_last_error": (ctypes.c_char_p, []),
+    "lkgpu_build_id": (ctypes.c_char_p, []),
     "lkgpu_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
     "lkgpu_context_destroy": (None, [_vp]),
     "lkgpu_set_chol_config": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
@@ -57,12 +60,18 @@ _lib = None
 
 
 def lib():
-    """Load liblkgpu.so; raises if it has not been built."""
+    """Load liblkgpu.so; raises if it has not been built, or was built from
+    sources other than those beside it."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m libkriging_amd.build`")
         L = ctypes.CDLL(LIB_PATH)
+        L.lkgpu_build_id.restype = ctypes.c_char_p
+        built, now = L.lkgpu_build_id().decode(), source_hash()
+        if now is not None and built != now:
+            raise ImportError(f"{LIB_PATH} was built from other sources (build id {built}, sources {now}): "
+                              "rebuild it with `python -m libkriging_amd.build`")
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
             fn.restype = res

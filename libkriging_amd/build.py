@@ -9,6 +9,8 @@ import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
 
+from ._srchash import source_hash
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
@@ -43,10 +45,10 @@ COMMON = [
 ]
 
 
-def _compile(src):
+def _compile(src, defines=()):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, "-x", "hip", *COMMON, "-c", path, "-o", obj]
+    cmd = [HIPCC, "-x", "hip", *COMMON, *defines, "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -56,8 +58,11 @@ def _compile(src):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    # the C ABI translation unit carries the hash of the sources, which the
+    # loader checks against the sources beside the library
+    bid = [f'-DLKGPU_BUILD_ID="{source_hash()}"']
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda s: _compile(s, bid if s == "lk_capi.cpp" else ()), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

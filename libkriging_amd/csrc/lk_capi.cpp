@@ -32,9 +32,15 @@ namespace lk {
 void set_last_error(const std::string& s) { g_last_error = s; }
 }  // namespace lk
 
+#ifndef LKGPU_BUILD_ID
+#define LKGPU_BUILD_ID "unknown"
+#endif
+
 extern "C" {
 
 const char* lk_get_last_error(void) { return g_last_error.c_str(); }
+
+const char* lkgpu_build_id(void) { return LKGPU_BUILD_ID; }
 
 int lkgpu_context_create(int device, lkgpu_context** out) {
   return guarded([&] {

@@ -1202,9 +1202,9 @@ hipError_t launch_mfma_probe(int blocks, int iters, int chains, double* out, hip
 template <int DT>
 __global__ __launch_bounds__(TPB) void k_predict_cov(PredictArgs a) {
   // lower tiles only; an off-diagonal tile also writes its transpose
-  int it, jt;
-  tri_decode(blockIdx.x, it, jt);
-  const int i0 = it * NB, j0 = jt * NB;
+  int ti, tj;
+  tri_decode(blockIdx.x, ti, tj);
+  const int i0 = ti * NB, j0 = tj * NB;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const int m = a.m, d = a.d;
   __shared__ double sbuf[4 * KC * LDSS];
@@ -1242,7 +1242,7 @@ __global__ __launch_bounds__(TPB) void k_predict_cov(PredictArgs a) {
         for (int q = 0; q < a.p; ++q) ec = fma(a.ecirc[(size_t)q * m + gi], a.ecirc[(size_t)q * m + gj], ec);
         const double v = acc[bi][bj][e] + rnn + ec;
         a.cov[(size_t)gj * m + gi] = v;
-        if (it != jt) a.cov[(size_t)gi * m + gj] = v;
+        if (ti != tj) a.cov[(size_t)gi * m + gj] = v;
       }
 }
 
