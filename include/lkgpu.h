@@ -141,6 +141,21 @@ int lkgpu_set_graph(lkgpu_context* ctx, int enable);
  * are the same).  Runs with a timed stage use one stream. */
 int lkgpu_set_streams(lkgpu_context* ctx, int streams);
 
+/* Size-bucket launch table (no reference counterpart).  json_text:
+ *   {"buckets": [{"max_nblk": B, "panel_blocks": P, "streams": S}, ...]}
+ * with B the bucket's largest matrix in 64-row blocks (the last bucket is
+ * unbounded), P the Cholesky panel width in block columns and S the parts a
+ * batch of that bucket alone is split into (1, 2, 4).  NULL restores the
+ * defaults (P = 4, S = 4).  A context loads tuning_gfx950.json from the
+ * library's directory at creation (or the file LKGPU_TUNING names; "none"
+ * skips it), the table libkriging_amd/autotune.py writes.  A batch whose
+ * slots span several buckets runs one part per bucket, each on its own
+ * stream with launches sized to its own largest matrix; results do not
+ * depend on the table (each slot's arithmetic is the same).
+ * lkgpu_last_parts returns the parts of the last evaluation. */
+int lkgpu_set_tuning(lkgpu_context* ctx, const char* json_text);
+int lkgpu_last_parts(lkgpu_context* ctx);
+
 /* How gradient evaluations form R^-1 (no reference counterpart; both give the
  * reference's LL, gradient and condition-check decision):
  *   0 (default) L^-1 by recursive doubling, then R^-1 = L^-T L^-1 in one launch
@@ -339,6 +354,12 @@ int lk_nested_kriging_predict_fallbacks(void* ptr);
 /* LL+grad evaluations and batched pipelines of the groups' optimisation in
  * the last fit (all groups share each pipeline); no reference counterpart. */
 int lk_nested_kriging_fit_statistics(void* ptr, int* evaluations, int* batches);
+/* The groups' own fitted hyper-parameters before unification
+ * (NestedKriging.cpp:308-322 combines them): theta_out nb_groups x d
+ * (group-major: group g at theta_out[g d ..]), sigma2_out and beta0_out
+ * nb_groups each (beta0 0 without a constant trend); any may be NULL.  No
+ * reference counterpart: the input of the unification, for checking it. */
+int lk_nested_kriging_group_fits(void* ptr, double* theta_out, double* sigma2_out, double* beta0_out);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,8 @@ SIGNATURES = {
     "lkgpu_set_chol_config": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
     "lkgpu_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lkgpu_set_streams": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lkgpu_set_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "lkgpu_last_parts": (ctypes.c_int, [_vp]),
     "lkgpu_problem_create": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, _ip]),
     "lkgpu_problem_destroy": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -48,7 +50,7 @@ SIGNATURES = {
 
 STAGES = {"build": 0, "cholesky": 1, "update0": 2, "trtri": 3, "solves": 4, "adjoint": 5}
 
-READ_L, READ_LINV, READ_FSTAR, READ_YSTAR, READ_ESTAR, READ_X = range(6)
+READ_L, READ_LINV, READ_FSTAR, READ_YSTAR, READ_ESTAR, READ_X, READ_RSTAR = range(7)
 
 
 class ModelFlags(ctypes.Structure):
@@ -132,6 +134,16 @@ class Context:
         """most streams (1, 2, 4) a batch is split over"""
         check(lib().lkgpu_set_streams(self.h, int(streams)))
 
+    def set_tuning(self, table):
+        """size-bucket launch table: a dict {"buckets": [...]}, its JSON
+        text, or None for the defaults"""
+        import json
+        txt = None if table is None else (table if isinstance(table, str) else json.dumps(table))
+        check(lib().lkgpu_set_tuning(self.h, None if txt is None else txt.encode()))
+
+    def last_parts(self):
+        return lib().lkgpu_last_parts(self.h)
+
     def problem(self, X, y, F, kernel, noise=None):
         X = f64(X)
         if X.ndim == 1:
@@ -210,10 +222,14 @@ class Context:
     def read_item(self, item, what):
         """Host copy of a factor of batch item `item` of the last batch."""
         n, d, p = self._last_shape[item]
+        if what == READ_RSTAR:
+            out = np.zeros((p, p), order="F")
+            check(lib().lkgpu_read_item(self.h, int(item), int(what), ptr(out)))
+            return out
         cols = {READ_L: n, READ_LINV: n, READ_FSTAR: p}.get(what, 1)
         out = np.zeros((n, cols), order="F")
         check(lib().lkgpu_read_item(self.h, int(item), int(what), ptr(out)))
-        return out if cols > 1 else out[:, 0]
+        return out if what in (READ_L, READ_LINV, READ_FSTAR) else out[:, 0]
 
 
 _default_ctx = None

@@ -246,8 +246,7 @@ void Kriging::logLikelihoodFunBatch(int nb, const double* gamma, int gamma_n, bo
   rt_->evaluate(reqs, want_grad, res);
   for (int i = 0; i < nb; ++i) {
     if (res[i].status != EVAL_OK)
-      throw std::runtime_error("[ERROR] Exceed max numerical nugget (" + std::to_string(res[i].inc) +
-                               " x 1e-10) added to force chol matrix");
+      throw std::runtime_error(chol_failure_message(res[i].status, res[i].inc, rt_->chol.num_nugget));
     ll[i] = res[i].ll;
     if (want_grad)
       for (int k = 0; k < gamma_n; ++k) grad[(size_t)i * gamma_n + k] = res[i].grad[k];
@@ -268,8 +267,7 @@ double Kriging::leaveOneOutFun(const double* theta, int theta_n, bool want_grad,
   std::vector<EvalResult> res;
   rt_->evaluate(reqs, want_grad, res, nullptr, OBJ_LOO);
   if (res[0].status != EVAL_OK)
-    throw std::runtime_error("[ERROR] Exceed max numerical nugget (" + std::to_string(res[0].inc) +
-                             " x 1e-10) added to force chol matrix");
+    throw std::runtime_error(chol_failure_message(res[0].status, res[0].inc, rt_->chol.num_nugget));
   if (want_grad)
     for (int k = 0; k < theta_n; ++k) grad[k] = res[0].grad[k];
   return res[0].ll;
@@ -311,8 +309,7 @@ double Kriging::logMargPostFun(const double* gamma, int gamma_n, bool want_grad,
   std::vector<EvalResult> res;
   rt_->evaluate(reqs, want_grad, res, nullptr, OBJ_LMP);
   if (res[0].status != EVAL_OK)
-    throw std::runtime_error("[ERROR] Exceed max numerical nugget (" + std::to_string(res[0].inc) +
-                             " x 1e-10) added to force chol matrix");
+    throw std::runtime_error(chol_failure_message(res[0].status, res[0].inc, rt_->chol.num_nugget));
   if (want_grad)
     for (int k = 0; k < gamma_n; ++k) grad[k] = res[0].grad[k];
   return res[0].ll;
@@ -439,7 +436,7 @@ struct FitState : FitJob {
         // (Kriging.cpp:2075-2081)
         w.done = true;
         w.success = false;
-        w.error = "Exceed max numerical nugget added to force chol matrix";
+        w.error = chol_failure_message(res[s].status, res[s].inc, m->rt_->chol.num_nugget);
         continue;
       }
       const double f = sign() * res[s].ll;
@@ -1011,7 +1008,7 @@ void Kriging::commit(const std::vector<double>& theta, double alpha) {
 
 void Kriging::commit_apply(const std::vector<double>& theta, double alpha, int slot, const EvalResult& r) {
   const int p = p_;
-  if (r.status != EVAL_OK) throw std::runtime_error("[ERROR] Exceed max numerical nugget added to force chol matrix");
+  if (r.status != EVAL_OK) throw std::runtime_error(chol_failure_message(r.status, r.inc, rt_->chol.num_nugget));
   theta_ = theta;
   const Runtime::SlotView v = rt_->slot_view(slot);
   npad_ = v.npad;
@@ -1045,8 +1042,9 @@ void Kriging::commit_apply(const std::vector<double>& theta, double alpha, int s
       for (int a = 0; a < p; ++a) s -= M[i + (size_t)a * n_] * beta_[a];
       z[i] = s;
     }
-    hip_check(hipMemcpyAsync(d_vec_ + (size_t)npad_ * p, z.data(), sizeof(double) * npad_, hipMemcpyHostToDevice,
-                             rt_->stream()),
+    // z is local: copy before it goes out of scope (the stream is idle here:
+    // read_slot synchronised it)
+    hip_check(hipMemcpy(d_vec_ + (size_t)npad_ * p, z.data(), sizeof(double) * npad_, hipMemcpyHostToDevice),
               "upload z");
   }
   Rstar_.assign((size_t)p * p, 0.0);

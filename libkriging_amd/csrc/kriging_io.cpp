@@ -64,7 +64,8 @@ void Kriging::save(const std::string& filename) {
   reqs[0].flags.alpha = alpha_;
   std::vector<EvalResult> res;
   rt_->evaluate(reqs, false, res);
-  if (res[0].status != EVAL_OK) throw std::runtime_error("[ERROR] Exceed max numerical nugget added to force chol matrix");
+  if (res[0].status != EVAL_OK)
+    throw std::runtime_error(chol_failure_message(res[0].status, res[0].inc, rt_->chol.num_nugget));
   std::vector<double> T((size_t)n * n), Rinv((size_t)n * n), M((size_t)n * std::max(p, 1)), z(n);
   rt_->read_slot(0, 0, T.data());
   if (p > 0) rt_->read_slot(0, 2, M.data());

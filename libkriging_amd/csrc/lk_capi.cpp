@@ -144,6 +144,21 @@ int lkgpu_set_streams(lkgpu_context* ctx, int streams) {
   });
 }
 
+int lkgpu_set_tuning(lkgpu_context* ctx, const char* json_text) {
+  return guarded([&] {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    if (json_text == nullptr) {
+      std::vector<lk::BucketTuning> t;
+      for (int b : {4, 8, 16, 32, 64, 128, 1 << 30}) t.push_back(lk::BucketTuning{b, 4, 4});
+      ctx->rt.set_tuning(t);
+    } else {
+      ctx->rt.load_tuning_json(json_text);
+    }
+  });
+}
+
+int lkgpu_last_parts(lkgpu_context* ctx) { return ctx ? ctx->rt.last_parts() : -1; }
+
 int lkgpu_gradient_path(lkgpu_context* ctx) {
   int v = -1;
   guarded([&] {

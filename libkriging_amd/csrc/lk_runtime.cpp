@@ -1,15 +1,30 @@
 #include "lk_runtime.hpp"
 
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <climits>
+#include <fstream>
+#include <sstream>
+#include <cmath>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
 
+#include "model_io.hpp"
+
 namespace lk {
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess) throw HipError(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+std::string chol_failure_message(int status, int inc, double num_nugget) {
+  if (status == EVAL_NUGGET_NOT_POSITIVE)
+    return "[ERROR] Cannot add numerical nugget which is not strictly positive: " + std::to_string(num_nugget);
+  return "[ERROR] Exceed max numerical nugget (" + std::to_string(inc) + " x 1e" +
+         std::to_string(std::log10(num_nugget)) + ") added to force chol matrix";
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -20,7 +35,6 @@ Runtime::Runtime(int device) : device_(device) {
   if (device < 0 || device >= count) throw std::runtime_error("no HIP device " + std::to_string(device));
   hip_check(hipSetDevice(device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
-  if (const char* e = std::getenv("LKGPU_PANEL_BLOCKS")) panel_blocks_ = std::max(1, std::atoi(e));
   if (const char* e = std::getenv("LKGPU_GRAD_PATH")) rinv_path_ = std::atoi(e) != 0;
   int S = 4;
   if (const char* e = std::getenv("LKGPU_STREAMS")) {
@@ -28,6 +42,74 @@ Runtime::Runtime(int device) : device_(device) {
     S = (v == 2 || v == 4) ? v : 1;
   }
   set_streams(S);
+  // size buckets of 4, 8, ..., 128 blocks and the rest; panel width and
+  // stream parts from the tuned table when one is found, else 4 and 4
+  std::vector<BucketTuning> t;
+  for (int b : {4, 8, 16, 32, 64, 128, INT_MAX}) t.push_back(BucketTuning{b, 4, 4});
+  set_tuning(t);
+  std::string path;
+  if (const char* e = std::getenv("LKGPU_TUNING")) {
+    path = e;
+  } else {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&hip_check), &info) != 0 && info.dli_fname != nullptr) {
+      std::string so = info.dli_fname;
+      const size_t slash = so.rfind('/');
+      path = (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/tuning_gfx950.json";
+    }
+  }
+  if (!path.empty() && path != "none") {
+    std::ifstream f(path);
+    if (f) {
+      std::stringstream ss;
+      ss << f.rdbuf();
+      load_tuning_json(ss.str());
+    }
+  }
+  // one panel width for every bucket (experiments)
+  if (const char* e = std::getenv("LKGPU_PANEL_BLOCKS")) {
+    const int pb = std::max(1, std::atoi(e));
+    for (BucketTuning& b : tuning_) b.panel_blocks = pb;
+  }
+}
+
+void Runtime::set_tuning(const std::vector<BucketTuning>& t) {
+  if (t.empty()) throw std::invalid_argument("tuning table must have at least one bucket");
+  std::vector<BucketTuning> v(t);
+  std::sort(v.begin(), v.end(), [](const BucketTuning& a, const BucketTuning& b) { return a.max_nblk < b.max_nblk; });
+  for (const BucketTuning& b : v) {
+    if (b.max_nblk < 1) throw std::invalid_argument("bucket bound must be positive");
+    if (b.panel_blocks < 1 || b.panel_blocks > 64) throw std::invalid_argument("panel_blocks must be in [1, 64]");
+    if (b.streams != 1 && b.streams != 2 && b.streams != 4) throw std::invalid_argument("streams must be 1, 2 or 4");
+  }
+  v.back().max_nblk = INT_MAX;
+  tuning_ = v;
+  slot_problem_.clear();  // the next batch recomputes its bucket layout
+}
+
+void Runtime::load_tuning_json(const std::string& text) {
+  const io::Json j = io::parse(text);
+  std::vector<BucketTuning> t;
+  for (const io::Json& b : j.at("buckets").arr) {
+    BucketTuning q;
+    const double mx = b.at("max_nblk").type == io::Json::Null ? (double)INT_MAX : b.at("max_nblk").as_number();
+    q.max_nblk = mx >= (double)INT_MAX ? INT_MAX : (int)mx;
+    q.panel_blocks = (int)b.at("panel_blocks").as_number();
+    q.streams = (int)b.at("streams").as_number();
+    t.push_back(q);
+  }
+  set_tuning(t);
+}
+
+int Runtime::bucket_of(int nblk) const {
+  for (size_t q = 0; q < tuning_.size(); ++q)
+    if (nblk <= tuning_[q].max_nblk) return (int)q;
+  return (int)tuning_.size() - 1;
+}
+
+int Runtime::slot_of(int req) const {
+  if (req < 0 || req >= (int)slot_of_req_.size()) throw std::invalid_argument("item index out of range");
+  return slot_of_req_[req];
 }
 
 void Runtime::set_streams(int S) {
@@ -126,12 +208,22 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
   for (int s = 0; s < ns; ++s) mapping[s] = reqs[s].problem;
   if (mapping == slot_problem_ && ns == nslots_) return;
 
+  // slots in bucket order (stable): slot_of_req_[r] is request r's slot
+  std::vector<int> bucket(ns), order(ns);
+  for (int r = 0; r < ns; ++r) bucket[r] = bucket_of((problem(mapping[r]).n + NB - 1) / NB);
+  for (int r = 0; r < ns; ++r) order[r] = r;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bucket[a] < bucket[b]; });
+  std::vector<int> slot_of_req(ns);
+  for (int s = 0; s < ns; ++s) slot_of_req[order[s]] = s;
+  std::vector<int> slot_prob(ns);
+  for (int s = 0; s < ns; ++s) slot_prob[s] = mapping[order[s]];
+
   std::vector<size_t> off(ns);
   size_t total = 0;
   int nblk_max = 0, dmax = 0;
   h_slots_.assign(ns, Slot{});
   for (int s = 0; s < ns; ++s) {
-    const DeviceProblem& P = problem(mapping[s]);
+    const DeviceProblem& P = problem(slot_prob[s]);
     const int npad = (P.n + NB - 1) / NB * NB;
     const int nblk = npad / NB;
     off[s] = total;
@@ -151,7 +243,7 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
   }
   char* base = reinterpret_cast<char*>(pool_);
   for (int s = 0; s < ns; ++s) {
-    const DeviceProblem& P = problem(mapping[s]);
+    const DeviceProblem& P = problem(slot_prob[s]);
     const int npad = (P.n + NB - 1) / NB * NB;
     const int nblk = npad / NB;
     char* q = base + off[s];
@@ -189,6 +281,26 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
     slot_cap_ = cap;
   }
   hip_check(hipMemcpy(d_slots_, h_slots_.data(), sizeof(Slot) * ns, hipMemcpyHostToDevice), "upload slots");
+  // contiguous slot range of each bucket present; more than kMaxStreams
+  // buckets: the smallest adjacent ones merged
+  std::vector<Part> parts;
+  for (int s = 0; s < ns; ++s) {
+    const int bk = bucket[order[s]];
+    if (parts.empty() || bucket[order[parts.back().first]] != bk) parts.push_back(Part{s, 0, 0, 0, 4});
+    Part& q = parts.back();
+    ++q.count;
+    q.nblk_max = std::max(q.nblk_max, h_slots_[s].nblk);
+    q.dmax = std::max(q.dmax, h_slots_[s].d);
+  }
+  while ((int)parts.size() > kMaxStreams) {
+    parts[0].count += parts[1].count;
+    parts[0].nblk_max = std::max(parts[0].nblk_max, parts[1].nblk_max);
+    parts[0].dmax = std::max(parts[0].dmax, parts[1].dmax);
+    parts.erase(parts.begin() + 1);
+  }
+  for (Part& q : parts) q.panel_blocks = tuning_[bucket_of(q.nblk_max)].panel_blocks;
+  buckets_ = parts;
+  slot_of_req_ = slot_of_req;
   slot_problem_ = mapping;
   nslots_ = ns;
   nblk_max_ = nblk_max;
@@ -199,13 +311,14 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
 // The pipeline as an ordered list of launches, each tagged with the stages
 // it belongs to (bit s = stage s): 0 build, 1 Cholesky, 2 first trailing
 // update, 3 trtri, 4 triangular solves + LL, 5 fused R^-1 / adjoint.
-std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_grad, int objective) const {
+std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_grad, int objective,
+                                                   int panel_blocks) const {
   std::vector<Step> v;
   v.push_back({1u << 0, "build", [B] { return launch_build(B); }});
   // blocked right-looking Cholesky, two levels: panels of P block columns;
   // inside a panel each step updates only the panel's remaining columns,
   // then one trailing update with K = 64 P per panel
-  const int P = panel_blocks_;
+  const int P = panel_blocks;
   bool first_update = true;
   for (int K0 = 0; K0 < B.nblk_max; K0 += P) {
     const int K1 = std::min(K0 + P, B.nblk_max);
@@ -330,14 +443,48 @@ void Runtime::take_stage_time(double* avg_ms, int* count) {
   timed_count_ = 0;
 }
 
-// Parts the batch is split into: the largest S <= streams_ (4, 2) whose
-// parts keep the XCD-aware placement (slots a multiple of 8 S); 1 for timed
-// runs, whose stage events sit on the main stream.
-int Runtime::split_parts(bool timed, BenchMap* bench) const {
-  if (timed || bench != nullptr) return 1;
-  for (int S = streams_; S > 1; S /= 2)
-    if (nslots_ % (8 * S) == 0) return S;
-  return 1;
+// Parts of one run.  Timed runs (stage events on the main stream) and bench
+// runs: one part.  A batch spanning several size buckets: one part per
+// bucket.  A batch of one bucket: the largest S <= min(streams_, the
+// bucket's tuned streams) equal parts whose slot counts are multiples of 8
+// (each part keeps the XCD-aware placement).
+std::vector<Runtime::Part> Runtime::plan_parts(bool timed, BenchMap* bench) const {
+  Part all{0, nslots_, nblk_max_, dmax_, tuning_[bucket_of(nblk_max_)].panel_blocks};
+  if (timed || bench != nullptr) return {all};
+  if (buckets_.size() > 1 && streams_ > 1) {
+    std::vector<Part> v = buckets_;
+    while ((int)v.size() > streams_) {
+      v[0].count += v[1].count;
+      v[0].nblk_max = std::max(v[0].nblk_max, v[1].nblk_max);
+      v[0].dmax = std::max(v[0].dmax, v[1].dmax);
+      v[0].panel_blocks = tuning_[bucket_of(v[0].nblk_max)].panel_blocks;
+      v.erase(v.begin() + 1);
+    }
+    return v;
+  }
+  const int smax = std::min(streams_, tuning_[bucket_of(nblk_max_)].streams);
+  for (int S = smax; S > 1; S /= 2)
+    if (nslots_ % (8 * S) == 0) {
+      std::vector<Part> v(S, all);
+      for (int q = 0; q < S; ++q) {
+        v[q].first = q * (nslots_ / S);
+        v[q].count = nslots_ / S;
+      }
+      return v;
+    }
+  return {all};
+}
+
+Batch Runtime::part_batch(const Part& q, hipStream_t s) const {
+  Batch B;
+  B.slots = d_slots_ + q.first;
+  B.params = d_params_ + q.first;
+  B.out = d_out_ + q.first;
+  B.nslots = q.count;
+  B.nblk_max = q.nblk_max;
+  B.dmax = q.dmax;
+  B.stream = s;
+  return B;
 }
 
 // launches of the S parts, interleaved, between a fork from and a join back
@@ -358,33 +505,21 @@ void Runtime::issue_split(const std::vector<Step>* parts, int S) {
 }
 
 void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
-  Batch B;
-  B.slots = d_slots_;
-  B.params = d_params_;
-  B.out = d_out_;
-  B.nslots = nslots_;
-  B.nblk_max = nblk_max_;
-  B.dmax = dmax_;
-  B.stream = stream_;
   hip_check(hipMemcpyAsync(d_params_, h_params_, sizeof(SlotParams) * nslots_, hipMemcpyHostToDevice, stream_),
             "upload params");
-  const std::vector<Step> steps = pipeline_steps(B, want_grad, objective);
+  const Part all{0, nslots_, nblk_max_, dmax_, tuning_[bucket_of(nblk_max_)].panel_blocks};
+  const std::vector<Step> steps = pipeline_steps(part_batch(all, stream_), want_grad, objective, all.panel_blocks);
   const auto rng = stage_range(steps, timed_stage_);
   const bool timed = rng.first < rng.second;
-  const int S = split_parts(timed, bench);
+  const std::vector<Part> plan = plan_parts(timed, bench);
+  const int S = (int)plan.size();
   const bool split = S > 1;
+  last_parts_ = S;
   std::vector<Step> part[kMaxStreams];
   if (split) {
-    const int h = nslots_ / S;
-    for (int q = 0; q < S; ++q) {
-      Batch H = B;
-      H.nslots = h;
-      H.slots = B.slots + q * h;
-      H.params = B.params + q * h;
-      H.out = B.out + q * h;
-      H.stream = q == 0 ? stream_ : xstream_[q - 1];
-      part[q] = pipeline_steps(H, want_grad, objective);
-    }
+    for (int q = 0; q < S; ++q)
+      part[q] = pipeline_steps(part_batch(plan[q], q == 0 ? stream_ : xstream_[q - 1]), want_grad, objective,
+                               plan[q].panel_blocks);
   }
   if (bench != nullptr) {
     // uncaptured run with one event pair around the whole pipeline
@@ -487,7 +622,7 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
   for (int s = 0; s < ns; ++s) {
     const EvalRequest& R = reqs[s];
     const DeviceProblem& P = problem(R.problem);
-    SlotParams& Q = h_params_[s];
+    SlotParams& Q = h_params_[slot_of_req_[s]];
     std::memset(&Q, 0, sizeof(Q));
     for (int k = 0; k < P.d; ++k) Q.theta[k] = R.gamma[k];
     double alpha = (R.gamma_n > P.d) ? R.gamma[P.d] : R.flags.alpha;
@@ -515,9 +650,9 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
     if (rinv) recheck_rcond_bound(ns, bench);
     bool retry = false;
     for (int s = 0; s < ns; ++s) {
-      SlotParams& Q = h_params_[s];
+      SlotParams& Q = h_params_[slot_of_req_[s]];
       if (!Q.active) continue;
-      const SlotOut& O = h_out_[s];
+      const SlotOut& O = h_out_[slot_of_req_[s]];
       const DeviceProblem& P = problem(reqs[s].problem);
       EvalResult& E = res[s];
       if (O.status == ST_OK) {
@@ -578,7 +713,7 @@ void Runtime::add_lmp_prior(const std::vector<EvalRequest>& reqs, std::vector<Ev
     const double n = P.n;
     const double a = 0.2, b = 1.0 / std::pow(n, 1.0 / d) * (a + d);
     const bool nug = P.noise == NOISE_NUGGET;
-    const double alpha = h_params_[s].alpha;
+    const double alpha = h_params_[slot_of_req_[s]].alpha;
     double t = nug ? (1.0 - alpha) / alpha : 0.0;
     for (int k = 0; k < d; ++k) t += P.range[k] / std::pow(n, 1.0 / d) / reqs[s].gamma[k];
     E.ll += -b * t + a * std::log(t);
@@ -652,15 +787,16 @@ void Runtime::recheck_rcond_bound(int ns, BenchMap* bench) {
   }
 }
 
-Runtime::SlotView Runtime::slot_view(int s) const {
-  if (s < 0 || s >= nslots_) throw std::out_of_range("slot index");
-  const Slot& S = h_slots_[s];
+Runtime::SlotView Runtime::slot_view(int req) const {
+  if (req < 0 || req >= nslots_) throw std::out_of_range("slot index");
+  const Slot& S = h_slots_[slot_of(req)];
   return SlotView{S.A, S.W, S.V, S.n, S.npad, S.p};
 }
 
 void Runtime::read_rinv(int s, double* out) {
   if (last_rinv_) throw std::runtime_error("R^-1 needs an evaluation on the L^-1 path");
   if (s < 0 || s >= nslots_) throw std::invalid_argument("slot out of range");
+  // s is a request index; slot_view translates it
   hip_check(hipSetDevice(device_), "hipSetDevice");
   Batch B;
   B.slots = d_slots_;
@@ -717,7 +853,7 @@ void Runtime::read_slot(int s, int what, double* out) {
       copy_cols(v.V + (size_t)(what == 7 ? v_loo_err(v.p) : v_loo_s2(v.p)) * ld, 1);
       break;
     case 6: {  // R* (p x p, upper), stored by the GLS kernel
-      const Slot& S = h_slots_[s];
+      const Slot& S = h_slots_[slot_of(s)];
       if (v.p > 0)
         hip_check(hipMemcpy(out, S.red + red_rstar_off(S.nblk), sizeof(double) * v.p * v.p, hipMemcpyDeviceToHost),
                   "read R*");

@@ -31,6 +31,12 @@ struct CholConfig {
   double min_rcond = 1e-18;
 };
 
+// The reference's message for an evaluation whose factorisation failed
+// (LinearAlgebra.cpp:78-86): "Exceed max numerical nugget (inc x 1e<log10
+// num_nugget>) ..." or "Cannot add numerical nugget which is not strictly
+// positive: <num_nugget>".  status is an EvalStatus.
+std::string chol_failure_message(int status, int inc, double num_nugget);
+
 // Estimation flags of a model (Kriging::_logLikelihood, Kriging.cpp:214-245)
 struct ModelFlags {
   bool est_sigma2 = true;
@@ -72,6 +78,16 @@ struct DeviceProblem {
   double* y = nullptr;
   double* F = nullptr;
   std::vector<double> range;  // max - min of each input column (prior of the LMP objective)
+};
+
+// Launch parameters of one size bucket: the slots whose matrices have at most
+// max_nblk 64-row blocks.  Chosen per bucket by the autotuner
+// (libkriging_amd/autotune.py), which writes them to the table the runtime
+// loads at start (tuning_<arch>.json beside the library).
+struct BucketTuning {
+  int max_nblk = 0;       // upper bound of the bucket, in 64-row blocks
+  int panel_blocks = 4;   // block columns per Cholesky panel
+  int streams = 4;        // parts (1, 2, 4) a batch of this bucket alone is split into
 };
 
 // Per-evaluation timings filled when a bench map is requested (Bench::toc
@@ -124,6 +140,20 @@ class Runtime {
   void set_streams(int S);
   int streams() const { return streams_; }
 
+  // Size buckets.  A batch whose slots fall in more than one bucket runs
+  // as one part per bucket (at most 4, the smallest buckets merged), each on
+  // its own stream with launches sized to its own largest matrix: the small
+  // matrices of an uneven batch do not wait through the chain of the
+  // largest one's diagonal blocks.  Slots are ordered by bucket; results are
+  // indexed by request as always.
+  void set_tuning(const std::vector<BucketTuning>& t);
+  const std::vector<BucketTuning>& tuning() const { return tuning_; }
+  // load a table ({"buckets": [{"max_nblk", "panel_blocks", "streams"}, ...]})
+  void load_tuning_json(const std::string& text);
+  int bucket_of(int nblk) const;
+  // parts of the last run (slot ranges in bucket order) and their bucket
+  int last_parts() const { return last_parts_; }
+
   // Live timing of one pipeline stage with a HIP event pair recorded on the
   // pipeline stream (inside the captured graph when graphs are used).
   // Stage ids as LKGPU_STAGE_* in include/lkgpu.h; -1 disables.
@@ -134,6 +164,7 @@ class Runtime {
 
  private:
   void ensure_slots(const std::vector<EvalRequest>& reqs);
+  int slot_of(int req) const;  // slot of request `req` of the last evaluate()
   void add_lmp_prior(const std::vector<EvalRequest>& reqs, std::vector<EvalResult>& res, bool want_grad) const;
   void run_pipeline(bool want_grad, BenchMap* bench, int objective = OBJ_LL);
 
@@ -145,7 +176,13 @@ class Runtime {
   };
 
  private:
-  std::vector<Step> pipeline_steps(const Batch& B, bool want_grad, int objective) const;
+  std::vector<Step> pipeline_steps(const Batch& B, bool want_grad, int objective, int panel_blocks) const;
+  // a contiguous slot range run as one unit of launches
+  struct Part {
+    int first = 0, count = 0, nblk_max = 0, dmax = 0, panel_blocks = 4;
+  };
+  std::vector<Part> plan_parts(bool timed, BenchMap* bench) const;
+  Batch part_batch(const Part& q, hipStream_t s) const;
   hipGraph_t capture(const std::vector<Step>& v, size_t a, size_t b);
 
   int device_ = 0;
@@ -155,6 +192,10 @@ class Runtime {
 
   // slot workspace
   std::vector<int> slot_problem_;
+  std::vector<int> slot_of_req_;      // request index -> slot (slots ordered by bucket)
+  std::vector<Part> buckets_;         // slot ranges of the buckets present (<= kMaxStreams)
+  std::vector<BucketTuning> tuning_;  // ascending max_nblk, the last one unbounded
+  int last_parts_ = 1;
   std::vector<size_t> slot_bytes_;
   std::vector<Slot> h_slots_;
   Slot* d_slots_ = nullptr;
@@ -178,8 +219,6 @@ class Runtime {
   GraphEntry graphs_[2 * OBJ_COUNT];
   int last_objective_ = OBJ_LL;
   int timed_stage_ = -1;
-  // block columns per Cholesky panel (two-level blocking); LKGPU_PANEL_BLOCKS
-  int panel_blocks_ = 4;
   // gradient evaluations: 0 = L^-1 (trtri) + R^-1 = L^-T L^-1 (lauum,
   // default), 1 = R^-1 sweep without L^-1; LKGPU_GRAD_PATH or set_gradient_path
   bool rinv_path_ = false;
@@ -194,7 +233,6 @@ class Runtime {
   hipStream_t xstream_[kMaxStreams - 1] = {};  // streams of parts 1 .. S-1
   hipEvent_t fork_ = nullptr;
   hipEvent_t join_[kMaxStreams - 1] = {};
-  int split_parts(bool timed, BenchMap* bench) const;
   void issue_split(const std::vector<Step>* parts, int S);
   hipEvent_t tev_[2] = {nullptr, nullptr};
   double timed_sum_ms_ = 0.0;

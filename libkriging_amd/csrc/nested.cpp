@@ -284,9 +284,15 @@ void NestedKriging::unify(const std::string& objective) {
   const int p = (int)groups_.size(), d = d_, n = n_;
   std::vector<double> logt(d, 0.0);
   double s2 = 0.0, b0 = 0.0;
+  gtheta_.assign((size_t)p * d, 0.0);
+  gsigma2_.assign(p, 0.0);
+  gbeta0_.assign(p, 0.0);
   for (int g = 0; g < p; ++g) {
     const double w = (double)groups_[g].size() / (double)n;
     const std::vector<double>& th = sub_[g]->theta();
+    for (int k = 0; k < d; ++k) gtheta_[(size_t)g * d + k] = th[k];
+    gsigma2_[g] = sub_[g]->sigma2();
+    if (regmodel_ == RegModel::Constant) gbeta0_[g] = sub_[g]->beta()[0];
     for (int k = 0; k < d; ++k) logt[k] += w * std::log(th[k]);
     s2 += w * sub_[g]->sigma2();
     if (regmodel_ == RegModel::Constant) b0 += w * sub_[g]->beta()[0];

@@ -62,6 +62,11 @@ class NestedKriging {
   int predict_fallbacks() const { return fallbacks_; }  // points solved by LU / pinv in the last predict
   // evaluations and batched pipelines of the groups' optimisation in the last fit
   int fit_evaluations() const { return fit_evals_; }
+  // the groups' own fitted hyper-parameters, before unification: theta
+  // (nb_groups x d, group-major), sigma2 and the constant-trend beta0
+  const std::vector<double>& group_theta() const { return gtheta_; }
+  const std::vector<double>& group_sigma2() const { return gsigma2_; }
+  const std::vector<double>& group_beta0() const { return gbeta0_; }
   int fit_batches() const { return fit_batches_; }
   std::string summary() const;
 
@@ -89,6 +94,7 @@ class NestedKriging {
   std::vector<std::unique_ptr<Kriging>> sub_;
   std::vector<double> theta_;
   double sigma2_ = 0.0, beta0_ = 0.0;
+  std::vector<double> gtheta_, gsigma2_, gbeta0_;
   bool fitted_ = false;
   int fallbacks_ = 0;
   int fit_evals_ = 0, fit_batches_ = 0;

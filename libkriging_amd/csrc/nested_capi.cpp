@@ -227,6 +227,17 @@ int lk_nested_kriging_fit_statistics(void* ptr, int* evaluations, int* batches) 
   });
 }
 
+int lk_nested_kriging_group_fits(void* ptr, double* theta_out, double* sigma2_out, double* beta0_out) {
+  return guarded([&] {
+    lk::NestedKriging* k = NK(ptr);
+    const auto& t = k->group_theta();
+    if (t.empty()) throw std::runtime_error("NestedKriging is not fitted");
+    if (theta_out) std::memcpy(theta_out, t.data(), sizeof(double) * t.size());
+    if (sigma2_out) std::memcpy(sigma2_out, k->group_sigma2().data(), sizeof(double) * k->group_sigma2().size());
+    if (beta0_out) std::memcpy(beta0_out, k->group_beta0().data(), sizeof(double) * k->group_beta0().size());
+  });
+}
+
 int lk_nested_kriging_predict_fallbacks(void* ptr) {
   int v = -1;
   guarded([&] { v = NK(ptr)->predict_fallbacks(); });

@@ -41,6 +41,7 @@ N.register("lk_nested_kriging_submodel", _vp, [_vp, _i])
 N.register("lk_nested_kriging_set_predict_chunk", _i, [_vp, _i])
 N.register("lk_nested_kriging_predict_fallbacks", _i, [_vp])
 N.register("lk_nested_kriging_fit_statistics", _i, [_vp, _ip, _ip])
+N.register("lk_nested_kriging_group_fits", _i, [_vp, _dp, _dp, _dp])
 
 
 def _nparams(parameters, d):
@@ -203,6 +204,14 @@ class NestedKriging:
         e, b = ctypes.c_int(), ctypes.c_int()
         N.check(N.lib().lk_nested_kriging_fit_statistics(self._h, ctypes.byref(e), ctypes.byref(b)))
         return e.value, b.value
+
+    def group_fits(self):
+        """(theta (nb_groups x d), sigma2, beta0) each group's fit reached
+        before the hyper-parameters were unified"""
+        p, d = self.nb_groups(), self.theta().size
+        th, s2, b0 = np.empty((p, d)), np.empty(p), np.empty(p)
+        N.check(N.lib().lk_nested_kriging_group_fits(self._h, N.ptr(th), N.ptr(s2), N.ptr(b0)))
+        return th, s2, b0
 
     def predict_fallbacks(self):
         return N.lib().lk_nested_kriging_predict_fallbacks(self._h)

@@ -67,8 +67,22 @@ def _solve_km(KM, kM):
         return np.linalg.pinv(KM) @ kM, 2
 
 
-def predict_nk(X, y, groups, theta, sigma2, beta0, kernel, Xn, chunk=128, return_info=False, jitter=JITTER):
-    """jitter: the reference's 1e-10; other values only for the pins"""
+_PAIRS = None  # (X, groups, W, theta, kernel) for the pair workers
+
+
+def _pair_cross(ij):
+    X, groups, W, theta, kernel = _PAIRS
+    i, j = ij
+    Rij = O.corr_mat(kernel, X[groups[i]], X[groups[j]], theta)
+    return np.sum(W[i] * (Rij @ W[j]), axis=0)
+
+
+def predict_nk(X, y, groups, theta, sigma2, beta0, kernel, Xn, chunk=128, return_info=False, jitter=JITTER,
+               workers=1):
+    """jitter: the reference's 1e-10; other values only for the pins.
+    workers > 1: the group-pair sums of a chunk over that many forked
+    processes (same arithmetic per pair)"""
+    global _PAIRS
     X = np.asarray(X, dtype=np.float64)
     Xn = np.asarray(Xn, dtype=np.float64)
     Ls, alphas = precompute_nk(X, np.asarray(y, dtype=np.float64), groups, theta, beta0, kernel, jitter)
@@ -90,12 +104,18 @@ def predict_nk(X, y, groups, theta, sigma2, beta0, kernel, Xn, chunk=128, return
             M[g] = beta0 + C.T @ alphas[g]
             Kd[g] = np.sum(U * U, axis=0)
         cross = np.zeros((p, p, qc))
-        for i in range(p):
-            for j in range(i + 1, p):
-                Rij = O.corr_mat(kernel, X[groups[i]], X[groups[j]], theta)
-                cij = np.sum(W[i] * (Rij @ W[j]), axis=0)
-                cross[i, j] = cij
-                cross[j, i] = cij
+        pairs = [(i, j) for i in range(p) for j in range(i + 1, p)]
+        _PAIRS = (X, groups, W, theta, kernel)
+        if workers > 1:
+            import multiprocessing as mproc
+            with mproc.get_context("fork").Pool(workers) as pool:
+                sums = pool.map(_pair_cross, pairs, chunksize=max(1, len(pairs) // (16 * workers)))
+        else:
+            sums = [_pair_cross(ij) for ij in pairs]
+        _PAIRS = None
+        for (i, j), cij in zip(pairs, sums):
+            cross[i, j] = cij
+            cross[j, i] = cij
         for t in range(qc):
             KM = cross[:, :, t].copy()
             KM[np.diag_indices(p)] = Kd[:, t] + jitter

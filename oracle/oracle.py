@@ -34,6 +34,12 @@ class Problem(ctypes.Structure):
                 ("alpha", ctypes.c_double)]
 
 
+class Model(ctypes.Structure):
+    _fields_ = [("R", _dp), ("L", _dp), ("Rinv", _dp), ("Fstar", _dp), ("ystar", _dp), ("Rstar", _dp),
+                ("Estar", _dp), ("betahat", _dp), ("SSEstar", ctypes.c_double), ("inc_cond", ctypes.c_int),
+                ("rcond", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -65,6 +71,11 @@ def lib():
         L.lko_predict.restype = ctypes.c_int
         L.lko_predict.argtypes = [ctypes.POINTER(Problem), _dp, ctypes.c_double, ctypes.c_double,
                                   ctypes.POINTER(CholCfg), _dp, ctypes.c_int, _dp, _dp]
+        L.lko_populate_model.restype = ctypes.c_int
+        L.lko_populate_model.argtypes = [ctypes.POINTER(Problem), _dp, ctypes.c_double, ctypes.POINTER(CholCfg),
+                                         ctypes.POINTER(Model)]
+        L.lko_solve_upper_t.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int]
+        L.lko_solve_upper_t.restype = None
         L.lko_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
@@ -127,6 +138,33 @@ class OracleProblem:
         if rc != 0:
             raise RuntimeError(lib().lko_last_error().decode())
         return ll.value, grad, inc.value
+
+    def populate_model(self, theta, alpha=1.0, cfg=None):
+        """KrigingImpl::populate_Model (KrigingImpl.cpp:73-125): dict of R
+        (un-jittered), L, Rinv, Fstar, ystar, Rstar, Estar, betahat, SSEstar,
+        inc, rcond, and x = L^-T E* (Kriging.cpp:294)"""
+        theta = _f64(np.asarray(theta, dtype=np.float64).reshape(-1))
+        n, p = self.n, self.p
+        arr = {k: np.zeros((n, n), order="F") for k in ("R", "L", "Rinv")}
+        arr["Fstar"] = np.zeros((n, max(p, 1)), order="F")
+        arr["ystar"] = np.zeros(n)
+        arr["Rstar"] = np.zeros((max(p, 1), max(p, 1)), order="F")
+        arr["Estar"] = np.zeros(n)
+        arr["betahat"] = np.zeros(max(p, 1))
+        m = Model(*[_ptr(arr[k]) for k in ("R", "L", "Rinv", "Fstar", "ystar", "Rstar", "Estar", "betahat")],
+                  0.0, 0, 0.0)
+        rc = lib().lko_populate_model(ctypes.byref(self.s), _ptr(theta), float(alpha), ctypes.byref(cfg or default_cfg()),
+                                      ctypes.byref(m))
+        if rc != 0:
+            raise RuntimeError(lib().lko_last_error().decode())
+        x = arr["Estar"].copy()
+        lib().lko_solve_upper_t(_ptr(arr["L"]), n, _ptr(x), 1)
+        out = dict(arr)
+        out["Fstar"] = arr["Fstar"][:, :p]
+        out["Rstar"] = arr["Rstar"][:p, :p]
+        out["betahat"] = arr["betahat"][:p]
+        out.update(SSEstar=m.SSEstar, inc=m.inc_cond, rcond=m.rcond, x=x)
+        return out
 
     def build_R(self, theta, factor=1.0):
         theta = _f64(theta)

@@ -49,11 +49,9 @@ def mt19937_uniform(seed, count):
     return np.array(out)
 
 
-def fit_bfgs(X, y, kernel, theta0_rows, lower, upper, max_iter=20, pgtol=1e-3, factr=1e10, max_restart=10, m=5):
-    """Multistart BFGS from the given starting rows, without nugget.
-    Returns (best_theta, best_ll, per_start_objectives)."""
-    P = O.OracleProblem(X, y, kernel, "constant")
-    d = X.shape[1]
+def fit_one_start(P, th_start, lower, upper, max_iter=20, pgtol=1e-3, factr=1e10, max_restart=10, m=5):
+    """One start of the BFGS loop with its restarts (Kriging.cpp:1950-2041):
+    returns (objective -LL at the kept point, theta there)."""
     glo, gup = np.log(lower), np.log(upper)
 
     def fofn(g):
@@ -61,31 +59,37 @@ def fit_bfgs(X, y, kernel, theta0_rows, lower, upper, max_iter=20, pgtol=1e-3, f
         ll, gr, _ = P.log_likelihood(th)
         return -ll, -(gr * th)
 
-    results = []
-    for th_start in np.atleast_2d(theta0_rows):
-        gamma = np.log(th_start)
-        lo = np.minimum(gamma, glo)
-        up = np.maximum(gamma, gup)
-        best_f, best_g = np.inf, gamma.copy()
-        retry = 0
-        while retry <= max_restart:
-            x, f, info = fmin_l_bfgs_b(fofn, gamma, bounds=list(zip(lo, up)), m=m, factr=factr, pgtol=pgtol,
-                                       maxiter=max_iter)
-            gamma = x
-            if f < best_f:
-                best_f, best_g = f, x.copy()
-            sol_to_lb = np.min(np.abs(np.exp(x) - lower))
-            task = info["task"] if isinstance(info["task"], str) else info["task"].decode()
-            if retry < max_restart and (task.startswith("ABNORMAL_TERMINATION_IN_LNSRCH") or info["nit"] <= 2
-                                        or sol_to_lb < EPS or f > best_f):
-                rth = (th_start + lower) / 2.0 ** (retry + 1)
-                gamma = np.log(rth)
-                lo = np.minimum(gamma, lo)
-                up = np.maximum(gamma, up)
-                retry += 1
-            else:
-                break
-        ll, _, _ = P.log_likelihood(np.exp(best_g), want_grad=False)
-        results.append((-ll, np.exp(best_g)))
+    gamma = np.log(th_start)
+    lo = np.minimum(gamma, glo)
+    up = np.maximum(gamma, gup)
+    best_f, best_g = np.inf, gamma.copy()
+    retry = 0
+    while retry <= max_restart:
+        x, f, info = fmin_l_bfgs_b(fofn, gamma, bounds=list(zip(lo, up)), m=m, factr=factr, pgtol=pgtol,
+                                   maxiter=max_iter)
+        gamma = x
+        if f < best_f:
+            best_f, best_g = f, x.copy()
+        sol_to_lb = np.min(np.abs(np.exp(x) - lower))
+        task = info["task"] if isinstance(info["task"], str) else info["task"].decode()
+        if retry < max_restart and (task.startswith("ABNORMAL_TERMINATION_IN_LNSRCH") or info["nit"] <= 2
+                                    or sol_to_lb < EPS or f > best_f):
+            rth = (th_start + lower) / 2.0 ** (retry + 1)
+            gamma = np.log(rth)
+            lo = np.minimum(gamma, lo)
+            up = np.maximum(gamma, up)
+            retry += 1
+        else:
+            break
+    ll, _, _ = P.log_likelihood(np.exp(best_g), want_grad=False)
+    return -ll, np.exp(best_g)
+
+
+def fit_bfgs(X, y, kernel, theta0_rows, lower, upper, max_iter=20, pgtol=1e-3, factr=1e10, max_restart=10, m=5):
+    """Multistart BFGS from the given starting rows, without nugget.
+    Returns (best_theta, best_ll, per_start_objectives)."""
+    P = O.OracleProblem(X, y, kernel, "constant")
+    results = [fit_one_start(P, th, lower, upper, max_iter, pgtol, factr, max_restart, m)
+               for th in np.atleast_2d(theta0_rows)]
     k = int(np.argmin([r[0] for r in results]))
     return results[k][1], -results[k][0], [r[0] for r in results]

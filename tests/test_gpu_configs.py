@@ -3,8 +3,8 @@ checked through properties that do not need the (cubic, single-threaded)
 oracle: an independent dense numpy/LAPACK formulation of the LL and of the
 analytic gradient (W = x x'/sigma2 - R^-1 contracted with dR/dtheta_k,
 Kriging.cpp:291-326), and central finite differences of the GPU's own LL.
-Tolerances scale with cond(R) where R is badly conditioned (two FP64 codes
-then agree to O(cond * eps))."""
+The dense formulation is held to the plain bars (LL 1e-10 relative,
+gradient 1e-8 normwise); the measured errors are printed."""
 import numpy as np
 import pytest
 
@@ -92,9 +92,10 @@ def test_single_evaluation_config_n4096():
     ll, g, st, inc = ctx.ll_grad_batch([pid], theta[None, :], True)
     assert st[0] == 0 and inc[0] == 0
     ll_d, g_d, c = dense_ll_grad(X, y, F, theta, "matern5_2")
-    print("n4096: ll", ll[0], ll_d, "cond", c, "grad rel", rel_err(g[0], g_d))
-    assert abs(ll[0] - ll_d) <= max(1e-10, 100 * c * EPS) * abs(ll_d)
-    assert rel_err(g[0], g_d) <= max(1e-8, 1000 * c * EPS)
+    e_ll, e_g = abs(ll[0] - ll_d) / abs(ll_d), rel_err(g[0], g_d)
+    print(f"n4096: LL {ll[0]!r} dense {ll_d!r} rel err {e_ll:.2e}; grad rel err {e_g:.2e}; cond(R) {c:.2e}")
+    assert e_ll <= 1e-10
+    assert e_g <= 1e-8
     fd = fd_grad(lambda t: ctx.ll_grad_batch([pid], t[None, :], False)[0][0], theta, 1e-6)
     assert rel_err(g[0], fd) <= 1e-5
 
@@ -102,10 +103,13 @@ def test_single_evaluation_config_n4096():
 def test_adjoint_nugget_config_n8192():
     """n = 8192, d = 10, gauss, theta ~ log-U[0.3, 1.5], nugget ratio 1e-6
     (alpha = 1 / (1 + 1e-6)); y = GP sample + N(0, 1e-3) noise.  Gradient over
-    the 10 thetas and alpha from the fused adjoint kernel, against central
-    finite differences (relative step 1e-6 in theta; in gamma_alpha =
-    -log(1 + 1e-3 - alpha) for alpha so the step cannot cross alpha = 1) and
-    the dense analytic gradient."""
+    the 10 thetas and alpha from the fused adjoint kernel, behind the model's
+    logLikelihoodFun and the torch.autograd.Function, against the dense
+    analytic gradient, central finite differences (relative step 1e-6 in
+    theta; in gamma_alpha = -log(1 + 1e-3 - alpha) for alpha so the step
+    cannot cross alpha = 1), and the layer-1 batch entry point."""
+    import torch
+    import libkriging_amd as lk
     from libkriging_amd import _native as N
     rng = np.random.default_rng(125)
     n, d = 8192, 10
@@ -124,10 +128,23 @@ def test_adjoint_nugget_config_n8192():
     gam = np.concatenate([theta, [alpha]])
     ll, g, st, inc = ctx.ll_grad_batch([pid], gam[None, :], True)
     assert st[0] == 0
+    # the model surface: logLikelihoodFun and the torch Function give the
+    # batch entry point's values bit for bit
+    k = lk.Kriging(y, X, "gauss", "constant", False, "none", "LL",
+                   {"theta": theta[None, :], "sigma2": 1.0, "nugget": 1e-6}, "nugget")
+    ll_k, g_k, _ = k.logLikelihoodFun(gam, True)
+    assert ll_k == ll[0] and np.array_equal(g_k, g[0])
+    t = torch.tensor(gam, dtype=torch.float64, requires_grad=True)
+    v = lk.log_likelihood(t, k)
+    v.backward()
+    assert v.item() == ll[0] and np.array_equal(t.grad.numpy(), g[0])
     ll_d, g_d, _ = dense_ll_grad(X, y, F, theta, "gauss", alpha)
-    print("n8192: ll", ll[0], ll_d, "inc", inc[0], "grad", g[0], "dense", g_d)
-    assert abs(ll[0] - ll_d) <= 1e-8 * abs(ll_d)
-    assert rel_err(g[0], g_d) <= 1e-6
+    e_ll, e_g = abs(ll[0] - ll_d) / abs(ll_d), rel_err(g[0], g_d)
+    print(f"n8192: LL {ll[0]!r} dense {ll_d!r} rel err {e_ll:.2e}; grad rel err {e_g:.2e} "
+          f"(theta part {rel_err(g[0][:d], g_d[:d]):.2e}, alpha {abs(g[0][d] - g_d[d]) / abs(g_d[d]):.2e}); "
+          f"jitter steps {inc[0]}")
+    assert e_ll <= 1e-10
+    assert e_g <= 1e-8
 
     def ll_of(gm):
         return ctx.ll_grad_batch([pid], gm[None, :], False)[0][0]

@@ -1,0 +1,198 @@
+"""The reference's numerical-robustness cases on the GPU path, and the
+KModel export.
+
+* tests/unstableLLTest.cpp:11-209: a 7-point 1-D design with a long-range
+  gauss kernel; the condition check on / off, num_nugget 1e-15, theta = 9
+  with sigma2 = 1e10 fixed, a BFGS fit, LL at theta = 5 with and without the
+  check, BFGS against BFGS10.
+* the retry ladder's decisions (LinearAlgebra.cpp:68-98): a factor rejected
+  by the condition check alone (potrf succeeds), both failure statuses
+  ("Exceed max numerical nugget", "not strictly positive") through the batch
+  entry point and through the model object.
+* lkgpu_read_item: L, L^-1, F*, y*, E*, x and R* of one evaluation against
+  the oracle's populate_Model (KrigingImpl.cpp:73-125).
+Decisions (jitter steps, statuses) are compared with the CPU oracle run
+under the same retry-ladder settings.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+from illcond_cases import near_duplicate
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from libkriging_amd import _native as N
+    c = N.Context(0)
+    yield c
+    c.close()
+
+
+def unstable_data():
+    X = np.array([0.0, 0.25, 0.33, 0.45, 0.5, 0.75, 1.0]).reshape(-1, 1)
+    x = X[:, 0]
+    y = 1.0 - 0.5 * (np.sin(4.0 * x) / (1.0 + x) + 2.0 * np.cos(12.0 * x) * x ** 6 + 0.7)
+    return X, y
+
+
+@pytest.fixture
+def linalg_defaults():
+    import libkriging_amd as lk
+    yield lk
+    lk.check_chol_rcond(True)
+    lk.set_num_nugget(1e-10)
+
+
+def test_unstable_fixed_theta_without_rcond_check(linalg_defaults):
+    lk = linalg_defaults
+    X, y = unstable_data()
+    lk.check_chol_rcond(False)
+    lk.set_num_nugget(1e-15)
+    k = lk.Kriging("gauss")
+    k.fit(y, X, "constant", False, "none", "LL",
+          {"theta": np.array([[9.0]]), "sigma2": 1e10, "is_theta_estim": False, "is_sigma2_estim": False})
+    assert abs(k.theta()[0] - 9.0) < 1e-10
+    assert abs(k.sigma2() - 1e10) < 1.0
+    ll, _, _ = k.logLikelihoodFun(np.array([9.0]), False)
+    P = O.OracleProblem(X, y, "gauss", est_sigma2=False, sigma2=1e10)
+    ll_o, _, inc_o = P.log_likelihood(np.array([9.0]), False, cfg=O.default_cfg(1e-15, False))
+    print(f"theta 9: LL {ll!r} oracle {ll_o!r} (oracle jitter steps {inc_o})")
+    assert np.isfinite(ll)
+    assert abs(ll - ll_o) <= 1e-6 * abs(ll_o)
+
+
+def test_unstable_bfgs_with_rcond_check(linalg_defaults):
+    lk = linalg_defaults
+    X, y = unstable_data()
+    lk.check_chol_rcond(True)
+    lk.set_num_nugget(1e-15)
+    k = lk.Kriging(y, X, "gauss", "constant", False, "BFGS", "LL")
+    print("BFGS theta", k.theta(), "LL", k.logLikelihood())
+    assert k.theta()[0] > 0.0
+    assert np.isfinite(k.logLikelihood())
+
+
+def test_unstable_ll_with_and_without_rcond_check(linalg_defaults, ctx):
+    lk = linalg_defaults
+    X, y = unstable_data()
+    lk.check_chol_rcond(True)
+    k = lk.Kriging(y, X, "gauss", "constant", False, "BFGS", "LL")
+    P = O.OracleProblem(X, y, "gauss")
+    pid = ctx.problem(X, y, np.ones((7, 1)), "gauss")
+    try:
+        for check in (True, False):
+            lk.check_chol_rcond(check)
+            ll, _, _ = k.logLikelihoodFun(np.array([5.0]), False)
+            assert np.isfinite(ll)
+            ctx.set_chol_config(1e-10, check)
+            ll1, _, st, inc = ctx.ll_grad_batch([pid], np.array([[5.0]]), False)
+            ll_o, _, inc_o = P.log_likelihood(np.array([5.0]), False, cfg=O.default_cfg(rcond_check=check))
+            print(f"rcond check {check}: LL {ll!r} (batch entry {ll1[0]!r}, jitter steps {inc[0]}), "
+                  f"oracle {ll_o!r} (jitter steps {inc_o})")
+            assert st[0] == 0 and ll1[0] == ll
+            assert inc[0] == inc_o
+    finally:
+        ctx.set_chol_config()
+        ctx.drop_problem(pid)
+
+
+def test_unstable_bfgs_vs_bfgs10(linalg_defaults):
+    lk = linalg_defaults
+    X, y = unstable_data()
+    lk.check_chol_rcond(True)
+    lk.set_num_nugget(1e-15)
+    k1 = lk.Kriging(y, X, "gauss", "constant", False, "BFGS", "LL")
+    k10 = lk.Kriging(y, X, "gauss", "constant", False, "BFGS10", "LL")
+    print("BFGS", k1.theta(), k1.logLikelihood(), "BFGS10", k10.theta(), k10.logLikelihood())
+    assert np.all(np.abs(k1.theta() - k10.theta()) <= 1e-4)
+    assert abs(k1.logLikelihood() - k10.logLikelihood()) < 1e-2
+
+
+def test_factor_rejected_by_the_condition_check_alone(ctx):
+    """potrf succeeds, rcond(L)^2 = 8.5e-14 is below a 1e-12 threshold: the
+    ladder adds one jitter step with the check on, none with it off"""
+    X, y = near_duplicate()
+    P = O.OracleProblem(X, y, "matern5_2")
+    L, info = O.potrf_lower(P.build_R([0.2, 0.2]))
+    rc2 = O.trcon_lower(np.tril(L)) ** 2
+    assert info == 0 and 1e-15 < rc2 < 1e-12  # the premise, on the CPU
+    pid = ctx.problem(X, y, np.ones((30, 1)), "matern5_2")
+    try:
+        for check, want in ((True, 1), (False, 0)):
+            ctx.set_chol_config(1e-10, check, 10, 1e-12)
+            _, _, st, inc = ctx.ll_grad_batch([pid], np.array([[0.2, 0.2]]), True)
+            _, _, inc_o = P.log_likelihood(np.array([0.2, 0.2]), cfg=O.default_cfg(1e-10, check, 10, 1e-12))
+            print(f"check {check}: jitter steps GPU {inc[0]} oracle {inc_o}")
+            assert st[0] == 0 and inc[0] == inc_o == want
+    finally:
+        ctx.set_chol_config()
+        ctx.drop_problem(pid)
+
+
+def _duplicates():
+    rng = np.random.default_rng(9)
+    X = rng.uniform(size=(80, 2))
+    X = np.vstack([X, X[:5]])
+    return X, np.sin(2 * np.pi * X).sum(axis=1)
+
+
+@pytest.mark.parametrize("num_nugget,status,message", [(1e-300, 1, "Exceed max numerical nugget"),
+                                                       (0.0, 2, "not strictly positive")])
+def test_failure_statuses(ctx, linalg_defaults, num_nugget, status, message):
+    """a singular R that no allowed jitter repairs: the batch entry point
+    reports the status per item, the model object raises the reference's
+    message, the oracle fails the same way"""
+    lk = linalg_defaults
+    X, y = _duplicates()
+    pid = ctx.problem(X, y, np.ones((85, 1)), "gauss")
+    pid2 = ctx.problem(X[:80], y[:80], np.ones((80, 1)), "matern5_2")  # distinct points
+    try:
+        ctx.set_chol_config(num_nugget, True, 3)
+        ll, _, st, inc = ctx.ll_grad_batch([pid, pid2], np.array([[0.3, 0.3], [0.1, 0.1]]), True)
+    finally:
+        ctx.set_chol_config()
+        ctx.drop_problem(pid)
+        ctx.drop_problem(pid2)
+    print("statuses", st, "jitter steps", inc)
+    assert st[0] == status
+    assert st[1] == 0  # the other item of the batch is unaffected
+    with pytest.raises(RuntimeError, match=message):
+        O.OracleProblem(X, y, "gauss").log_likelihood(np.array([0.3, 0.3]),
+                                                       cfg=O.default_cfg(num_nugget, True, 3))
+    k = lk.Kriging(y, X, "gauss", "constant", False, "none", "LL", {"theta": np.array([[0.05, 0.05]])})
+    lk.set_num_nugget(num_nugget)
+    with pytest.raises(RuntimeError, match=message if status == 2 else "numerical nugget"):
+        k.logLikelihoodFun(np.array([0.3, 0.3]), True)
+
+
+def test_read_item_matches_populate_model(ctx):
+    from libkriging_amd import _native as N
+    rng = np.random.default_rng(41)
+    X = rng.uniform(size=(150, 3))
+    y = np.sin(3 * X).sum(axis=1) + X[:, 0]
+    F = O.trend_matrix(X, "linear")
+    pid = ctx.problem(X, y, F, "matern5_2")
+    gam = np.array([[0.3, 0.4, 0.5], [0.5, 0.35, 0.25]])
+    try:
+        ll, _, st, _ = ctx.ll_grad_batch([pid] * 2, gam, False)
+        assert (st == 0).all()
+        P = O.OracleProblem(X, y, "matern5_2", "linear")
+        for item in range(2):
+            m = P.populate_model(gam[item])
+            L = np.tril(m["L"])
+            ref = {N.READ_L: L, N.READ_LINV: np.linalg.solve(L, np.eye(150)), N.READ_FSTAR: m["Fstar"],
+                   N.READ_YSTAR: m["ystar"], N.READ_ESTAR: m["Estar"], N.READ_X: m["x"], N.READ_RSTAR: m["Rstar"]}
+            for what, r in ref.items():
+                got = ctx.read_item(item, what)
+                e = rel_err(got, r)
+                print(f"item {item} buffer {what}: rel err {e:.2e}")
+                assert got.shape == r.shape
+                assert e <= 1e-12
+            # L and L^-1 come back with exact zeros above the diagonal
+            assert np.all(np.triu(ctx.read_item(item, N.READ_L), 1) == 0.0)
+    finally:
+        ctx.drop_problem(pid)

@@ -264,3 +264,86 @@ def test_predict_interpolates_and_matches_numpy():
     var = 1 - np.sum(r * (Ri @ r), axis=0) + (u ** 2).ravel() / (F.T @ Ri @ F).item()
     assert np.allclose(m, mean, rtol=1e-9)
     assert np.allclose(s, np.sqrt(1.3 * np.maximum(var, 0)), rtol=1e-6)
+
+
+def _numpy_corr(X, kernel, theta):
+    D = np.abs(X[:, None, :] - X[None, :, :]) / theta
+    if kernel == "gauss":
+        C = np.exp(-0.5 * np.sum(D ** 2, axis=2))
+    elif kernel == "exp":
+        C = np.exp(-np.sum(D, axis=2))
+    elif kernel == "matern3_2":
+        v = np.sqrt(3) * D
+        C = np.prod((1 + v) * np.exp(-v), axis=2)
+    else:
+        v = np.sqrt(5) * D
+        C = np.prod((1 + v + v ** 2 / 3) * np.exp(-v), axis=2)
+    np.fill_diagonal(C, 1.0)
+    return C
+
+
+def _gauss_loglik(y, F, K):
+    """log N(y; F beta, K) with beta the GLS estimate under K: a dense
+    scipy formulation that knows nothing of alpha or of the model's branches"""
+    from scipy.linalg import cho_factor, cho_solve
+    cf = cho_factor(K, lower=True)
+    KiF = cho_solve(cf, F)
+    beta = np.linalg.solve(F.T @ KiF, KiF.T @ y)
+    r = y - F @ beta
+    n = len(y)
+    return -0.5 * (n * np.log(2 * np.pi) + 2 * np.sum(np.log(np.diag(cf[0]))) + r @ cho_solve(cf, r))
+
+
+# (est_sigma2, est_nugget): the nugget value branches of Kriging.cpp:247-268.
+# For each, the covariance the model implies, from the parameter values alone:
+#   sigma2 fixed, nugget estimated: tau2 = sigma2 (1 - alpha) / alpha
+#   sigma2 estimated, nugget fixed: sigma2 = tau2 alpha / (1 - alpha)
+#   both fixed: alpha = sigma2 / (sigma2 + tau2) (Kriging.cpp:241-242)
+# and the LL is the Gaussian log-density of y under K = sigma2 C + tau2 I
+@pytest.mark.parametrize("kernel", ["gauss", "matern5_2"])
+@pytest.mark.parametrize("branch", ["sigma2_fixed", "nugget_fixed", "both_fixed"])
+def test_nugget_value_branches_match_gaussian_density(kernel, branch):
+    rng = np.random.default_rng(21)
+    X = rng.uniform(size=(35, 2))
+    y = np.cos(4 * X[:, 0]) + X[:, 1] + rng.normal(scale=0.1, size=35)
+    theta, alpha, s2, tau2 = np.array([0.3, 0.45]), 0.8, 1.7, 0.05
+    C = _numpy_corr(X, kernel, theta)
+    F = np.ones((35, 1))
+    if branch == "sigma2_fixed":
+        P = O.OracleProblem(X, y, kernel, "constant", "nugget", est_sigma2=False, sigma2=s2)
+        ll, _, _ = P.log_likelihood(np.array([*theta, alpha]), want_grad=False)
+        K = s2 * C + s2 * (1 - alpha) / alpha * np.eye(35)
+    elif branch == "nugget_fixed":
+        P = O.OracleProblem(X, y, kernel, "constant", "nugget", est_nugget=False, nugget=tau2)
+        ll, _, _ = P.log_likelihood(np.array([*theta, alpha]), want_grad=False)
+        K = tau2 * alpha / (1 - alpha) * C + tau2 * np.eye(35)
+    else:
+        P = O.OracleProblem(X, y, kernel, "constant", "nugget", est_sigma2=False, sigma2=s2, est_nugget=False,
+                            nugget=tau2)
+        ll, _, _ = P.log_likelihood(theta, want_grad=False)
+        K = s2 * C + tau2 * np.eye(35)
+    assert rel_err(ll, _gauss_loglik(y, F, K)) < 1e-11
+
+
+def test_nugget_branch_pin_catches_swapped_variance():
+    # a plausible slip -- sigma2 = tau2 (1 - alpha) / alpha instead of
+    # tau2 alpha / (1 - alpha) -- changes the density the pin computes
+    rng = np.random.default_rng(21)
+    X = rng.uniform(size=(35, 2))
+    y = np.cos(4 * X[:, 0]) + X[:, 1] + rng.normal(scale=0.1, size=35)
+    theta, alpha, tau2 = np.array([0.3, 0.45]), 0.8, 0.05
+    C = _numpy_corr(X, "gauss", theta)
+    P = O.OracleProblem(X, y, "gauss", "constant", "nugget", est_nugget=False, nugget=tau2)
+    ll, _, _ = P.log_likelihood(np.array([*theta, alpha]), want_grad=False)
+    wrong = _gauss_loglik(y, np.ones((35, 1)), tau2 * (1 - alpha) / alpha * C + tau2 * np.eye(35))
+    assert rel_err(ll, wrong) > 1e-3
+
+
+def test_noise_free_fixed_sigma2_matches_gaussian_density():
+    rng = np.random.default_rng(22)
+    X = rng.uniform(size=(30, 3))
+    y = np.sin(3 * X).sum(axis=1)
+    theta = np.array([0.4, 0.3, 0.6])
+    P = O.OracleProblem(X, y, "exp", "linear", est_sigma2=False, sigma2=0.8)
+    ll, _, _ = P.log_likelihood(theta, want_grad=False)
+    assert rel_err(ll, _gauss_loglik(y, O.trend_matrix(X, "linear"), 0.8 * _numpy_corr(X, "exp", theta))) < 1e-11
