@@ -94,6 +94,26 @@ int lkgpu_ll_grad_batch(lkgpu_context* ctx, int nb, const int* problem_ids, cons
                         const lkgpu_model_flags* flags, int flags_n, int return_grad, double* ll_out,
                         double* grad_out, int* status_out, int* inc_out);
 
+/* lkgpu_ll_grad_batch without allocating.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code around HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Same
+ * arguments, results and retry ladder as lkgpu_ll_grad_batch; the designs
+ * are the ones lkgpu_problem_create uploaded beforehand.  The slot tables and
+ * every item's workspace (2 npad^2 + npad (2 p + 13) doubles plus the
+ * reduction partials, npad = n rounded up to 64) live in a static device
+ * array of 64 MiB per GPU, held until the process ends; a batch that needs
+ * more, or has more than 64 items, fails with -1 and a message naming the
+ * sizes (e.g. one item up to n = 1984, 48 items of n = 256).  The call makes
+ * no device or pinned-host allocation and instantiates no graph: launches
+ * go directly to the context's stream.  Every call on one GPU uses that
+ * GPU's array, which keeps the last call's contents: calls that overlap on
+ * one GPU (two threads, two contexts) race.  Nothing in the array needs
+ * clearing: every value is written before it is read.  lkgpu_read_item
+ * reads the items of the last call, whichever entry point made it. */
+int lkgpu_ll_grad_batch_noalloc(lkgpu_context* ctx, int nb, const int* problem_ids, const double* gamma,
+                                int gamma_n, const lkgpu_model_flags* flags, int flags_n, int return_grad,
+                                double* ll_out, double* grad_out, int* status_out, int* inc_out);
+
 /* Copy a factor of batch item `item` of the last lkgpu_ll_grad_batch call
  * to host memory (column-major, leading dimension n): the KModel fields of
  * the reference (src/lib/include/libKriging/KrigingImpl.hpp:26-38).

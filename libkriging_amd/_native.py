@@ -42,6 +42,8 @@ SIGNATURES = {
     "lkgpu_problem_destroy": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lkgpu_ll_grad_batch": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, ctypes.c_int, _vp, ctypes.c_int,
                                            ctypes.c_int, _dp, _dp, _ip, _ip]),
+    "lkgpu_ll_grad_batch_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, ctypes.c_int, _vp, ctypes.c_int,
+                                                   ctypes.c_int, _dp, _dp, _ip, _ip]),
     "lkgpu_read_item": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     "lkgpu_synchronize": (ctypes.c_int, [_vp]),
     "lkgpu_set_timed_stage": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -162,8 +164,10 @@ class Context:
         check(lib().lkgpu_problem_destroy(self.h, int(pid)))
         self._problems.pop(pid, None)
 
-    def ll_grad_batch(self, problem_ids, gamma, return_grad=True, flags=None):
+    def ll_grad_batch(self, problem_ids, gamma, return_grad=True, flags=None, noalloc=False):
         """gamma: (nb, gamma_n).  flags: ModelFlags or list of them.
+        noalloc: through lkgpu_ll_grad_batch_noalloc (the device's static
+        scratch array, no allocation).
         Returns (ll[nb], grad[nb, gamma_n] or None, status[nb], inc[nb])."""
         gamma = np.ascontiguousarray(np.atleast_2d(np.asarray(gamma, dtype=np.float64)))
         nb, gn = gamma.shape
@@ -180,7 +184,8 @@ class Context:
         grad = np.zeros((nb, gn)) if return_grad else None
         st = np.zeros(nb, dtype=np.int32)
         inc = np.zeros(nb, dtype=np.int32)
-        check(lib().lkgpu_ll_grad_batch(
+        entry = lib().lkgpu_ll_grad_batch_noalloc if noalloc else lib().lkgpu_ll_grad_batch
+        check(entry(
             self.h, nb, ids.ctypes.data_as(_ip), gamma.ctypes.data_as(_dp), gn,
             ctypes.cast(fl, _vp) if fl is not None else None, fn, int(return_grad),
             ll.ctypes.data_as(_dp), grad.ctypes.data_as(_dp) if return_grad else None,

@@ -21,6 +21,7 @@ SOURCES = [
     "lk_nested.hip",
     "lk_rinv.hip",
     "lk_loo.hip",
+    "lk_noalloc.hip",
     "lk_runtime.cpp",
     "lk_capi.cpp",
     "lbfgsb.cpp",

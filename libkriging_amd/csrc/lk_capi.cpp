@@ -85,9 +85,10 @@ int lkgpu_problem_destroy(lkgpu_context* ctx, int id) {
   });
 }
 
-int lkgpu_ll_grad_batch(lkgpu_context* ctx, int nb, const int* problem_ids, const double* gamma, int gamma_n,
-                        const lkgpu_model_flags* flags, int flags_n, int return_grad, double* ll_out,
-                        double* grad_out, int* status_out, int* inc_out) {
+namespace {
+int ll_grad_batch(lkgpu_context* ctx, int nb, const int* problem_ids, const double* gamma, int gamma_n,
+                  const lkgpu_model_flags* flags, int flags_n, int return_grad, double* ll_out, double* grad_out,
+                  int* status_out, int* inc_out, bool noalloc) {
   return guarded([&] {
     if (ctx == nullptr || problem_ids == nullptr || gamma == nullptr || ll_out == nullptr)
       throw std::invalid_argument("NULL argument");
@@ -110,7 +111,10 @@ int lkgpu_ll_grad_batch(lkgpu_context* ctx, int nb, const int* problem_ids, cons
       }
     }
     std::vector<lk::EvalResult> res;
-    ctx->rt.evaluate(reqs, return_grad != 0, res);
+    if (noalloc)
+      ctx->rt.evaluate_noalloc(reqs, return_grad != 0, res);
+    else
+      ctx->rt.evaluate(reqs, return_grad != 0, res);
     for (int i = 0; i < nb; ++i) {
       ll_out[i] = res[i].ll;
       if (return_grad)
@@ -119,6 +123,21 @@ int lkgpu_ll_grad_batch(lkgpu_context* ctx, int nb, const int* problem_ids, cons
       if (inc_out) inc_out[i] = res[i].inc;
     }
   });
+}
+}  // namespace
+
+int lkgpu_ll_grad_batch(lkgpu_context* ctx, int nb, const int* problem_ids, const double* gamma, int gamma_n,
+                        const lkgpu_model_flags* flags, int flags_n, int return_grad, double* ll_out,
+                        double* grad_out, int* status_out, int* inc_out) {
+  return ll_grad_batch(ctx, nb, problem_ids, gamma, gamma_n, flags, flags_n, return_grad, ll_out, grad_out,
+                       status_out, inc_out, false);
+}
+
+int lkgpu_ll_grad_batch_noalloc(lkgpu_context* ctx, int nb, const int* problem_ids, const double* gamma,
+                                int gamma_n, const lkgpu_model_flags* flags, int flags_n, int return_grad,
+                                double* ll_out, double* grad_out, int* status_out, int* inc_out) {
+  return ll_grad_batch(ctx, nb, problem_ids, gamma, gamma_n, flags, flags_n, return_grad, ll_out, grad_out,
+                       status_out, inc_out, true);
 }
 
 int lkgpu_synchronize(lkgpu_context* ctx) {

@@ -143,6 +143,14 @@ hipError_t launch_rcond_estimate(const Batch& b);
 
 int trtri_levels(int nblk);
 
+// Allocation-free evaluations: the static device scratch array (64 MiB per
+// device) that holds the slot tables and workspaces, and the most items one
+// call may carry.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+constexpr int NOALLOC_MAX_ITEMS = 64;
+// device address of the scratch array on `device` (the current device)
+hipError_t noalloc_scratch(int device, char** ptr);
+
 // Prediction at new points from a committed model (KrigingImpl::predict_impl,
 // src/lib/KrigingImpl.cpp:145-243): mean and variance in normalised units.
 struct PredictArgs {

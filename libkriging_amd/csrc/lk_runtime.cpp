@@ -35,6 +35,9 @@ Runtime::Runtime(int device) : device_(device) {
   if (device < 0 || device >= count) throw std::runtime_error("no HIP device " + std::to_string(device));
   hip_check(hipSetDevice(device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  // the allocation-free entry point's scratch: its device address, looked
+  // up once per device with this device current
+  hip_check(noalloc_scratch(device, &scratch_), "hipGetSymbolAddress(scratch)");
   if (const char* e = std::getenv("LKGPU_GRAD_PATH")) rinv_path_ = std::atoi(e) != 0;
   int S = 4;
   if (const char* e = std::getenv("LKGPU_STREAMS")) {
@@ -235,13 +238,31 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
   }
   hip_check(hipSetDevice(device_), "hipSetDevice");
   hip_check(hipStreamSynchronize(stream_), "sync");
-  if (total > pool_bytes_) {
+  char* base = nullptr;
+  if (noalloc_) {
+    // slot tables first, then the workspaces, all in the scratch array
+    const size_t t1 = align256(sizeof(Slot) * ns), t2 = align256(sizeof(SlotParams) * ns),
+                 t3 = align256(sizeof(SlotOut) * ns);
+    if (ns > NOALLOC_MAX_ITEMS)
+      throw std::invalid_argument("the allocation-free entry point takes at most " +
+                                  std::to_string(NOALLOC_MAX_ITEMS) + " items, got " + std::to_string(ns));
+    if (t1 + t2 + t3 + total > NOALLOC_SCRATCH_BYTES)
+      throw std::invalid_argument("this batch needs " + std::to_string(t1 + t2 + t3 + total) +
+                                  " bytes of workspace; the allocation-free entry point has " +
+                                  std::to_string(NOALLOC_SCRATCH_BYTES) + " (use lkgpu_ll_grad_batch)");
+    d_slots_ = reinterpret_cast<Slot*>(scratch_);
+    d_params_ = reinterpret_cast<SlotParams*>(scratch_ + t1);
+    d_out_ = reinterpret_cast<SlotOut*>(scratch_ + t1 + t2);
+    h_params_ = np_params_;
+    h_out_ = np_out_;
+    base = scratch_ + t1 + t2 + t3;
+  } else if (total > pool_bytes_) {
     if (pool_) hip_check(hipFree(pool_), "hipFree pool");
     pool_ = nullptr;
     hip_check(hipMalloc(&pool_, total), "hipMalloc slot pool");
     pool_bytes_ = total;
   }
-  char* base = reinterpret_cast<char*>(pool_);
+  if (!noalloc_) base = reinterpret_cast<char*>(pool_);
   for (int s = 0; s < ns; ++s) {
     const DeviceProblem& P = problem(slot_prob[s]);
     const int npad = (P.n + NB - 1) / NB * NB;
@@ -266,7 +287,7 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
     S.kernel = P.kernel;
     S.noise = P.noise;
   }
-  if ((size_t)ns > slot_cap_) {
+  if (!noalloc_ && (size_t)ns > slot_cap_) {
     if (d_slots_) hipFree(d_slots_);
     if (d_params_) hipFree(d_params_);
     if (d_out_) hipFree(d_out_);
@@ -450,7 +471,7 @@ void Runtime::take_stage_time(double* avg_ms, int* count) {
 // (each part keeps the XCD-aware placement).
 std::vector<Runtime::Part> Runtime::plan_parts(bool timed, BenchMap* bench) const {
   Part all{0, nslots_, nblk_max_, dmax_, tuning_[bucket_of(nblk_max_)].panel_blocks};
-  if (timed || bench != nullptr) return {all};
+  if (timed || bench != nullptr || noalloc_) return {all};
   if (buckets_.size() > 1 && streams_ > 1) {
     std::vector<Part> v = buckets_;
     while ((int)v.size() > streams_) {
@@ -535,7 +556,7 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
     (*bench)["gpu pipeline"] += ms;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-  } else if (use_graph) {
+  } else if (use_graph && !noalloc_) {
     if (objective < 0 || objective >= OBJ_COUNT) throw std::invalid_argument("unknown objective");
     GraphEntry& G = graphs_[2 * objective + (want_grad ? 1 : 0)];
     if (G.version != layout_version_ || G.exec == nullptr || G.timed_stage != timed_stage_ || G.split != S) {
@@ -694,6 +715,38 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
   last_objective_ = objective;
   if (objective == OBJ_LMP) add_lmp_prior(reqs, res, want_grad);
   // leave every slot active for the next call's graph (params are rewritten)
+}
+
+void Runtime::evaluate_noalloc(const std::vector<EvalRequest>& reqs, bool want_grad,
+                               std::vector<EvalResult>& res) {
+  if (scratch_ == nullptr) throw std::runtime_error("no scratch array on this device");
+  // the allocating path's workspace, kept aside and restored
+  Slot* s_slots = d_slots_;
+  SlotParams* s_dp = d_params_;
+  SlotOut* s_do = d_out_;
+  SlotParams* s_hp = h_params_;
+  SlotOut* s_ho = h_out_;
+  auto restore = [&] {
+    d_slots_ = s_slots;
+    d_params_ = s_dp;
+    d_out_ = s_do;
+    h_params_ = s_hp;
+    h_out_ = s_ho;
+    noalloc_ = false;
+    // the next allocating call lays its slots out again (in its own pool)
+    slot_problem_.clear();
+    nslots_ = 0;
+  };
+  noalloc_ = true;
+  slot_problem_.clear();
+  nslots_ = 0;
+  try {
+    evaluate(reqs, want_grad, res, nullptr, OBJ_LL);
+  } catch (...) {
+    restore();
+    throw;
+  }
+  restore();
 }
 
 // The approximate reference prior of the LMP objective (Kriging.cpp:585-598,

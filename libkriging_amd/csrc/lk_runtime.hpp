@@ -113,6 +113,14 @@ class Runtime {
   void evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std::vector<EvalResult>& res,
                 BenchMap* bench = nullptr, int objective = OBJ_LL);
 
+  // The same LL evaluation with no device or pinned-host allocation: slot
+  // tables and workspaces in the device's static scratch array, parameters
+  // and results through host arrays of this object, launches issued
+  // directly (no graph), one stream.  Throws when the batch needs more than
+  // NOALLOC_SCRATCH_BYTES or has more than NOALLOC_MAX_ITEMS items.  The
+  // allocating path's workspace is kept and laid out again by its next call.
+  void evaluate_noalloc(const std::vector<EvalRequest>& reqs, bool want_grad, std::vector<EvalResult>& res);
+
   // Device pointers of slot s after the last evaluate(): the factor L (A,
   // lower, ld = npad), L^-1 (W lower) and the vector block V.
   struct SlotView {
@@ -196,6 +204,12 @@ class Runtime {
   std::vector<Part> buckets_;         // slot ranges of the buckets present (<= kMaxStreams)
   std::vector<BucketTuning> tuning_;  // ascending max_nblk, the last one unbounded
   int last_parts_ = 1;
+  // allocation-free calls: the device scratch array and the host arrays
+  // that stand in for the pinned parameter / result tables
+  char* scratch_ = nullptr;
+  bool noalloc_ = false;
+  SlotParams np_params_[NOALLOC_MAX_ITEMS] = {};
+  SlotOut np_out_[NOALLOC_MAX_ITEMS] = {};
   std::vector<size_t> slot_bytes_;
   std::vector<Slot> h_slots_;
   Slot* d_slots_ = nullptr;
