@@ -74,19 +74,6 @@ def lauum_traffic_bytes():
     return None
 
 
-def rinv_traffic_bytes():
-    """HBM bytes per batch of the R^-1 sweep stage (all its launches), from
-    the copy-calibrated counter passes summarised in profiles/"""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            for line in f:
-                if line.startswith("rinv_sweep traffic_bytes_per_batch"):
-                    return int(line.split()[-1])
-    except OSError:
-        pass
-    return None
-
-
 def mfma_probe_sweep(N):
     """FP64 MFMA issue rate from registers (lkgpu_mfma_probe) over independent
     chains per wave and workgroups (256 threads) per launch"""
@@ -266,18 +253,7 @@ def main():
     same = bool(l1[0] == ll[NSTART - 1] and np.array_equal(g1[0], grad[NSTART - 1]))
 
     # roofline of the live-timed kernel
-    if args.stage == "adjoint" and ctx.gradient_path() == 1:
-        # R^-1 from L by the backward block sweep: 2 n^3 / 3 flops per
-        # evaluation (the work of L^-1 and L^-T L^-1 together), all its
-        # launches plus the x part of the adjoint timed as one stage
-        flops = NSTART * 2.0 * n ** 3 / 3.0
-        achieved = flops / (stage_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / MFMA_FP64_PEAK_TFLOPS, "traffic": rinv_traffic_bytes(),
-                "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated (profiles/)",
-                "kernel": "k_zsweep_offdiag + k_zsweep_diag + k_adjoint_x", "avg_ms": stage_ms,
-                "launches": stage_count}
-    elif args.stage == "adjoint":
+    if args.stage == "adjoint":
         # R^-1 = L^-T L^-1 on MFMA: n^3/3 flops per evaluation
         flops = NSTART * n ** 3 / 3.0
         achieved = flops / (stage_ms * 1e-3) / 1e12
@@ -316,7 +292,6 @@ def main():
         "plain_equals_batched": same,
         "streams": 4,
         "split_equals_single_stream": split_equals_single,
-        "gradient_path": "R^-1 block sweep" if ctx.gradient_path() == 1 else "L^-1 + L^-T L^-1",
         "roofline": roof,
     }
     copy_ms = ctx.device_copy(COPY_BYTES, 5)

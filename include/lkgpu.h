@@ -176,18 +176,6 @@ int lkgpu_set_streams(lkgpu_context* ctx, int streams);
 int lkgpu_set_tuning(lkgpu_context* ctx, const char* json_text);
 int lkgpu_last_parts(lkgpu_context* ctx);
 
-/* How gradient evaluations form R^-1 (no reference counterpart; both give the
- * reference's LL, gradient and condition-check decision):
- *   0 (default) L^-1 by recursive doubling, then R^-1 = L^-T L^-1 in one launch
- *     over all lower tiles (the adjoint contraction fused);
- *   1 the backward block sweep R^-1 = Z from L^T Z = L^-1, solves by one
- *     product with Z, no L^-1 (one launch per block column: slower for
- *     batches of small matrices, kept as an alternative).
- * After a gradient evaluation on path 1, lkgpu_read_item serves L, x and R*
- * only (L^-1, F*, y*, E* are not formed).  Also set by LKGPU_GRAD_PATH. */
-int lkgpu_set_gradient_path(lkgpu_context* ctx, int path);
-int lkgpu_gradient_path(lkgpu_context* ctx);
-
 /* A plain device-to-device copy kernel over `bytes` (a multiple of 16),
  * repeated `reps` times; *ms_out = mean time per copy.  No counterpart in the
  * reference: the HBM bandwidth reference and the calibration of the

@@ -23,8 +23,6 @@ NOISES = {None: 0, "none": 0, "nugget": 1}
 # exported symbols and their (restype, argtypes); every one is declared in
 # include/lkgpu.h
 SIGNATURES = {
-    "lkgpu_set_gradient_path": (ctypes.c_int, [_vp, ctypes.c_int]),
-    "lkgpu_gradient_path": (ctypes.c_int, [_vp]),
     "lkgpu_device_copy": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "lkgpu_mfma_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double)]),
@@ -197,13 +195,6 @@ class Context:
 
     def set_timed_stage(self, stage):
         check(lib().lkgpu_set_timed_stage(self.h, STAGES[stage] if isinstance(stage, str) else int(stage)))
-
-    def set_gradient_path(self, path):
-        """1: R^-1 by the backward block sweep (default); 0: L^-1 then L^-T L^-1"""
-        check(lib().lkgpu_set_gradient_path(self.h, int(path)))
-
-    def gradient_path(self):
-        return lib().lkgpu_gradient_path(self.h)
 
     def device_copy(self, nbytes, reps=3):
         """mean ms of a plain device copy kernel over nbytes"""

@@ -19,7 +19,6 @@ LIB = os.path.join(HERE, "liblkgpu.so")
 SOURCES = [
     "lk_kernels.hip",
     "lk_nested.hip",
-    "lk_rinv.hip",
     "lk_loo.hip",
     "lk_noalloc.hip",
     "lk_runtime.cpp",

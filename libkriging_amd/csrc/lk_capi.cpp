@@ -148,13 +148,6 @@ int lkgpu_synchronize(lkgpu_context* ctx) {
   });
 }
 
-int lkgpu_set_gradient_path(lkgpu_context* ctx, int path) {
-  return guarded([&] {
-    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
-    if (path != 0 && path != 1) throw std::invalid_argument("gradient path must be 0 or 1");
-    ctx->rt.set_gradient_path(path);
-  });
-}
 
 int lkgpu_set_streams(lkgpu_context* ctx, int streams) {
   return guarded([&] {
@@ -178,14 +171,6 @@ int lkgpu_set_tuning(lkgpu_context* ctx, const char* json_text) {
 
 int lkgpu_last_parts(lkgpu_context* ctx) { return ctx ? ctx->rt.last_parts() : -1; }
 
-int lkgpu_gradient_path(lkgpu_context* ctx) {
-  int v = -1;
-  guarded([&] {
-    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
-    v = ctx->rt.gradient_path();
-  });
-  return v;
-}
 
 int lkgpu_device_copy(lkgpu_context* ctx, size_t bytes, int reps, double* ms_out) {
   return guarded([&] {

@@ -20,8 +20,6 @@ enum SlotStatus {
   ST_RCOND_SUSPECT = 2, // exact rcond below threshold: the 1-norm estimator decides
   ST_RCOND_LOW = 3,     // estimator confirmed rcond below threshold
   ST_INACTIVE = 4,
-  ST_RCOND_BOUND = 5,   // gradient path: the cheap bound on rcond is below the
-                        // threshold; the exact L^-1 path decides
 };
 
 // Static description of one batch slot: which design it evaluates and the
@@ -99,9 +97,15 @@ __host__ __device__ inline int red_grad_off(int nblk) { return 4 * nblk; }     /
 constexpr int GRAD_STRIDE = 2 * DMAX + 4;
 // R* = chol_upper(F*'F*), p x p column-major, after the adjoint partials
 __host__ __device__ inline int red_rstar_off(int nblk) { return 4 * nblk + nblk * (nblk + 1) / 2 * GRAD_STRIDE; }
-// two 64 x 64 scratch tiles of the R^-1 sweep, after R*
-__host__ __device__ inline int red_ztmp_off(int nblk) { return (red_rstar_off(nblk) + PMAX * PMAX + 64 + 31) & ~31; }
-__host__ __device__ inline size_t red_size(int nblk) { return (size_t)red_ztmp_off(nblk) + 2 * NB * NB; }
+// per diagonal block k, after R*: 64 max|L_kk| max|L_kk^-1|, a bound on
+// cond(L_kk) up to the factor 64 (written by the diagonal-block kernel, read
+// by the panel kernel)
+__host__ __device__ inline int red_cond_off(int nblk) { return red_rstar_off(nblk) + PMAX * PMAX; }
+__host__ __device__ inline size_t red_size(int nblk) { return (size_t)red_cond_off(nblk) + nblk; }
+// Panels whose diagonal block's condition estimate exceeds this get one step
+// of iterative refinement: multiplying by the explicit inverse of an
+// ill-conditioned block is not backward stable, the refined product is.
+constexpr double PANEL_REFINE_COND = 1e4;
 
 struct Batch {
   const Slot* slots;        // device
@@ -125,19 +129,9 @@ hipError_t launch_solve_fy(const Batch& b);
 hipError_t launch_gls(const Batch& b);
 hipError_t launch_solve_e(const Batch& b);
 hipError_t launch_solve_x(const Batch& b);
-hipError_t launch_finalize_ll(const Batch& b, bool rinv_path = false);
+hipError_t launch_finalize_ll(const Batch& b);
 hipError_t launch_lauum_adjoint(const Batch& b);
 
-// Gradient path without L^-1: R^-1 = Z by a backward block sweep from
-// L^T Z = L^-1 (the diagonal inverses L_jj^-1 come from potrf_diag), the
-// solves as one symmetric product with Z, the adjoint split into its trace
-// part (fused into the sweep) and its x part (k_adjoint_x).
-hipError_t launch_zsweep_offdiag(const Batch& b, int j);  // Z(i, j), i > j, and the P_j partials
-hipError_t launch_zsweep_diag(const Batch& b, int j);     // Z(j, j)
-hipError_t launch_zsymv(const Batch& b);                  // [Z F | Z y]
-hipError_t launch_gls_z(const Batch& b);                  // beta, x = Z (y - F beta), SSE
-hipError_t launch_norm1_L(const Batch& b);                // |L|_1 column maxima
-hipError_t launch_adjoint_x(const Batch& b);              // x' dR_k x terms
 hipError_t launch_finalize_grad(const Batch& b);
 hipError_t launch_rcond_estimate(const Batch& b);
 

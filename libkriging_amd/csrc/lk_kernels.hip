@@ -313,13 +313,18 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(const Slot* __restrict__ slo
     return;
   }
   // L_kk -> A (lower, zero above) and LDS
+  double lmax = 0.0;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = 16 * w + q;
     const double v = (l >= c) ? a[q] : 0.0;
     Ab[(size_t)c * ld + l] = v;
     sh.L[c * (NB + 2) + l] = v;
+    lmax = fmax(lmax, fabs(v));
   }
+  __shared__ double smax[2][4];
+  lmax = wave_max(lmax);
+  if (l == 0) smax[0][w] = lmax;
   __syncthreads();
   if (w == 0) {
     const double ld_part = wave_sum(log(sh.diag[l]));
@@ -333,16 +338,35 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(const Slot* __restrict__ slo
   }
   __syncthreads();
   // X rows staged as sh.X[row][col]; write columns coalesced (lane = row)
+  double xmax = 0.0;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = 16 * w + q;
-    Wb[(size_t)c * ld + l] = sh.X[l * (NB + 1) + c];
+    const double v = sh.X[l * (NB + 1) + c];
+    Wb[(size_t)c * ld + l] = v;
+    xmax = fmax(xmax, fabs(v));
+  }
+  xmax = wave_max(xmax);
+  if (l == 0) smax[1][w] = xmax;
+  __syncthreads();
+  // condition estimate of L_kk for the panel kernel's refinement decision
+  if (tid == 0) {
+    const double lm = fmax(fmax(smax[0][0], smax[0][1]), fmax(smax[0][2], smax[0][3]));
+    const double xm = fmax(fmax(smax[1][0], smax[1][1]), fmax(smax[1][2], smax[1][3]));
+    S.red[red_cond_off(S.nblk) + k] = (double)NB * lm * xm;
   }
   (void)sbad;
 }
 
 // ---------------------------------------------------------------------------
-// (2b) panel: L(r, k) = A(r, k) * L_kk^-T for row blocks r > k
+// (2b) panel: L(r, k) = A(r, k) * L_kk^-T for row blocks r > k, as a product
+// with the explicit inverse from the diagonal-block kernel.  When the
+// diagonal block is ill-conditioned (its estimate above PANEL_REFINE_COND)
+// the product is refined once, X += (A - X L_kk^T) L_kk^-T: the product
+// alone is accurate to cond(L_kk) eps relative to |L_kk^-1|, the refined one
+// to the level of a substitution (dtrsm, as LAPACK's dpotrf uses).  The
+// unrefined X waits in W's strict upper triangle (row block k, column block
+// r: unused until the inverse is formed) while the residual is formed.
 // grid (nblk_max - k - 1, nslots), 256 threads
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ slots,
@@ -360,7 +384,20 @@ __global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ sl
   acc_zero(acc);
   // A(i0 + ii, k) times Dinv(c, k) at W[k * ld + k0 + c]
   tile_mainloop<false, false, false>(S.A, ld, i0, S.W, ld, k0, k0, k0 + NB, sbuf, acc, tid, lane, wr, wc);
-  acc_store(acc, S.A + (size_t)k0 * ld + i0, ld, lane, wr, wc);
+  double* A21 = S.A + (size_t)k0 * ld + i0;
+  if (S.red[red_cond_off(S.nblk) + k] > PANEL_REFINE_COND) {
+    // X^T into W(k0 + c, i0 + r); the residual A21 - X L_kk^T into A21
+    acc_store_t(acc, S.W + (size_t)i0 * ld + k0, ld, lane, wr, wc);
+    d4 res[2][2];
+    acc_load(res, A21, ld, lane, wr, wc);
+    __syncthreads();
+    tile_mainloop<true, false, true>(S.W, ld, i0, S.A, ld, k0, k0, k0 + NB, sbuf, res, tid, lane, wr, wc);
+    acc_store(res, A21, ld, lane, wr, wc);
+    __syncthreads();
+    // X += residual Dinv^T
+    tile_mainloop<false, false, false>(S.A, ld, i0, S.W, ld, k0, k0, k0 + NB, sbuf, acc, tid, lane, wr, wc);
+  }
+  acc_store(acc, A21, ld, lane, wr, wc);
 }
 
 // ---------------------------------------------------------------------------
@@ -625,7 +662,7 @@ __global__ __launch_bounds__(MV_TPB) void k_solve_x(const Slot* __restrict__ slo
 // LinearAlgebra.cpp:68-115).  One thread per slot.
 // ---------------------------------------------------------------------------
 __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
-                              SlotOut* __restrict__ out, int nslots, int rinv_path) {
+                              SlotOut* __restrict__ out, int nslots) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nslots) return;
   const Slot S = slots[b];
@@ -640,9 +677,6 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
     nL = fmax(nL, S.red[red_colL_off(nb) + q]);
     nW = fmax(nW, S.red[red_colW_off(nb) + q]);
   }
-  // gradient path: red_colW holds max Z_ii = max |L^-1 e_i|_2^2 per block,
-  // and |L^-1|_1 <= sqrt(n) max_i |L^-1 e_i|_2 bounds the exact norm
-  if (rinv_path) nW = sqrt((double)S.n * nW);
   const double n = (double)S.n;
   const double two_pi = 2.0 * 3.14159265358979323846;
   O.logdet = logdet;
@@ -652,7 +686,7 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
   const double rc = (1.0 / nL) / nW;
   O.rcond2 = rc * rc;
   if (P.rcond_check && !(O.rcond2 >= P.min_rcond) && O.status == ST_OK)
-    O.status = rinv_path ? ST_RCOND_BOUND : ST_RCOND_SUSPECT;
+    O.status = ST_RCOND_SUSPECT;
   double ll, s2g;
   if (P.objective == OBJ_LMP) {
     // log marginal likelihood of Kriging::_logMargPost (Kriging.cpp:560-583),
@@ -1471,9 +1505,9 @@ hipError_t launch_solve_x(const Batch& B) {
   hipLaunchKernelGGL(k_solve_x, dim3(B.nblk_max, B.nslots), dim3(MV_TPB), 0, B.stream, B.slots, B.params, B.out);
   return hipGetLastError();
 }
-hipError_t launch_finalize_ll(const Batch& B, bool rinv_path) {
+hipError_t launch_finalize_ll(const Batch& B) {
   hipLaunchKernelGGL(k_finalize_ll, dim3((B.nslots + 63) / 64), dim3(64), 0, B.stream, B.slots, B.params, B.out,
-                     B.nslots, rinv_path ? 1 : 0);
+                     B.nslots);
   return hipGetLastError();
 }
 template <bool LMP>

@@ -38,7 +38,6 @@ Runtime::Runtime(int device) : device_(device) {
   // the allocation-free entry point's scratch: its device address, looked
   // up once per device with this device current
   hip_check(noalloc_scratch(device, &scratch_), "hipGetSymbolAddress(scratch)");
-  if (const char* e = std::getenv("LKGPU_GRAD_PATH")) rinv_path_ = std::atoi(e) != 0;
   int S = 4;
   if (const char* e = std::getenv("LKGPU_STREAMS")) {
     const int v = std::atoi(e);
@@ -358,22 +357,6 @@ std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_gra
       first_update = false;
     }
   }
-  if (want_grad && rinv_path_ && objective == OBJ_LL) {
-    // gradient without L^-1: R^-1 by the backward block sweep (trace part
-    // of the adjoint fused), solves by one symmetric product with R^-1
-    v.push_back({1u << 4, "norm1_L", [B] { return launch_norm1_L(B); }});
-    for (int j = B.nblk_max - 1; j >= 0; --j) {
-      if (j < B.nblk_max - 1)
-        v.push_back({1u << 5, "zsweep_offdiag", [B, j] { return launch_zsweep_offdiag(B, j); }});
-      v.push_back({1u << 5, "zsweep_diag", [B, j] { return launch_zsweep_diag(B, j); }});
-    }
-    v.push_back({1u << 4, "zsymv", [B] { return launch_zsymv(B); }});
-    v.push_back({1u << 4, "gls_z", [B] { return launch_gls_z(B); }});
-    v.push_back({1u << 4, "finalize_ll", [B] { return launch_finalize_ll(B, true); }});
-    v.push_back({1u << 5, "adjoint_x", [B] { return launch_adjoint_x(B); }});
-    v.push_back({0u, "finalize_grad", [B] { return launch_finalize_grad(B); }});
-    return v;
-  }
   const int lv = trtri_levels(B.nblk_max);
   for (int l = 0; l < lv; ++l) {
     v.push_back({1u << 3, "trtri step 0", [B, l] { return launch_trtri_level(B, 1 << l, 0); }});
@@ -665,10 +648,8 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
     Q.gamma_has_alpha = R.gamma_n > P.d;
     Q.objective = objective;
   }
-  const bool rinv = want_grad && rinv_path_ && objective == OBJ_LL;
   for (;;) {
     run_pipeline(want_grad, bench, objective);
-    if (rinv) recheck_rcond_bound(ns, bench);
     bool retry = false;
     for (int s = 0; s < ns; ++s) {
       SlotParams& Q = h_params_[slot_of_req_[s]];
@@ -711,7 +692,6 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
     }
     if (!retry) break;
   }
-  last_rinv_ = rinv;
   last_objective_ = objective;
   if (objective == OBJ_LMP) add_lmp_prior(reqs, res, want_grad);
   // leave every slot active for the next call's graph (params are rewritten)
@@ -809,37 +789,6 @@ void Runtime::add_lmp_prior(const std::vector<EvalRequest>& reqs, std::vector<Ev
   }
 }
 
-// Slots of an R^-1-sweep evaluation whose cheap rcond bound fell below the
-// threshold: one L^-1-path evaluation of just those slots (exact |L^-1|_1,
-// then the 1-norm estimator) decides as the reference's check does; the
-// sweep's LL and gradient are kept when it accepts.
-void Runtime::recheck_rcond_bound(int ns, BenchMap* bench) {
-  std::vector<int> bound;
-  for (int s = 0; s < ns; ++s)
-    if (h_params_[s].active && h_out_[s].status == ST_RCOND_BOUND) bound.push_back(s);
-  if (bound.empty()) return;
-  std::vector<SlotOut> saved(h_out_, h_out_ + ns);
-  std::vector<int> active(ns);
-  for (int s = 0; s < ns; ++s) {
-    active[s] = h_params_[s].active;
-    h_params_[s].active = 0;
-    h_params_[s].want_grad = 0;
-  }
-  for (int s : bound) h_params_[s].active = 1;
-  run_pipeline(false, bench);
-  std::vector<SlotOut> exact(h_out_, h_out_ + ns);
-  for (int s = 0; s < ns; ++s) {
-    h_out_[s] = saved[s];
-    h_params_[s].active = active[s];
-    h_params_[s].want_grad = 1;
-  }
-  for (int s : bound) {
-    h_out_[s].status = exact[s].status == ST_OK ? ST_OK : ST_RCOND_LOW;
-    h_out_[s].rcond2 = exact[s].rcond2;
-    h_out_[s].norm1_Linv = exact[s].norm1_Linv;
-  }
-}
-
 Runtime::SlotView Runtime::slot_view(int req) const {
   if (req < 0 || req >= nslots_) throw std::out_of_range("slot index");
   const Slot& S = h_slots_[slot_of(req)];
@@ -847,7 +796,6 @@ Runtime::SlotView Runtime::slot_view(int req) const {
 }
 
 void Runtime::read_rinv(int s, double* out) {
-  if (last_rinv_) throw std::runtime_error("R^-1 needs an evaluation on the L^-1 path");
   if (s < 0 || s >= nslots_) throw std::invalid_argument("slot out of range");
   // s is a request index; slot_view translates it
   hip_check(hipSetDevice(device_), "hipSetDevice");
@@ -869,9 +817,6 @@ void Runtime::read_rinv(int s, double* out) {
 
 void Runtime::read_slot(int s, int what, double* out) {
   const SlotView v = slot_view(s);
-  if (last_rinv_ && what >= 1 && what <= 4)
-    throw std::runtime_error("L^-1, F*, y* and E* are not formed by gradient evaluations (they take the R^-1 sweep); "
-                             "evaluate without the gradient first");
   const int n = v.n, ld = v.npad;
   hip_check(hipSetDevice(device_), "hipSetDevice");
   hip_check(hipStreamSynchronize(stream_), "sync");

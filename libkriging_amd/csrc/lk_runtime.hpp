@@ -139,9 +139,6 @@ class Runtime {
 
   CholConfig chol;
   bool use_graph = true;
-  void set_gradient_path(int path) { rinv_path_ = path != 0; }
-  void recheck_rcond_bound(int ns, BenchMap* bench);
-  int gradient_path() const { return rinv_path_ ? 1 : 0; }
   // most parts (1, 2 or 4) an untimed evaluation splits its batch into, one
   // stream each; a batch is split into the largest S <= this with slots a
   // multiple of 8 S.  Also set by LKGPU_STREAMS.
@@ -233,12 +230,6 @@ class Runtime {
   GraphEntry graphs_[2 * OBJ_COUNT];
   int last_objective_ = OBJ_LL;
   int timed_stage_ = -1;
-  // gradient evaluations: 0 = L^-1 (trtri) + R^-1 = L^-T L^-1 (lauum,
-  // default), 1 = R^-1 sweep without L^-1; LKGPU_GRAD_PATH or set_gradient_path
-  bool rinv_path_ = false;
-  // kind of the last run on the slots: L^-1, F*, y*, E* are formed only by
-  // evaluations that do not take the R^-1 sweep
-  bool last_rinv_ = false;
   // the batch split into S equal parts on S streams: the latency-bound
   // launches of one part (diagonal blocks, panels, finalisation) run beside
   // the MFMA launches of the others

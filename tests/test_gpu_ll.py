@@ -66,8 +66,9 @@ def _check(ctx, X, y, kernel, regmodel, noise, gammas, flags_kw=None, names=None
                 assert np.array_equal(Xc, X) and np.array_equal(gc, gam[b])
                 assert inc[b] == _TRUTH[name]["inc"]
                 e, ok = arbitrate(_TRUTH[name], ll[b], g[b], ll_r, g_r)
-                print(f"slot {b} ({name}): vs 50 digits LL {e['ll_gpu']:.2e} (oracle {e['ll_orc']:.2e}), "
-                      f"grad {e['g_gpu']:.2e} (oracle {e['g_orc']:.2e}), inc {inc[b]}")
+                print(f"slot {b} ({name}): vs 50 digits LL {e['ll_gpu']:.2e} (oracle {e['ll_orc']:.2e}, CPU "
+                      f"orderings max {e['ll_cpu_max']:.2e}), grad {e['g_gpu']:.2e} (oracle {e['g_orc']:.2e}, "
+                      f"max {e['g_cpu_max']:.2e}), inc {inc[b]}")
                 assert ok
                 continue
             print(f"slot {b}: ll {ll[b]!r} vs {ll_r!r} (rel {abs(ll[b] - ll_r) / abs(ll_r):.2e}), "
@@ -241,61 +242,6 @@ def test_batch_equals_single(ctx):
             assert np.array_equal(g1[0], gb[b])
     finally:
         ctx.drop_problem(pid)
-
-
-@pytest.mark.parametrize("kernel,noise,regmodel", [("gauss", None, "constant"), ("matern5_2", "nugget", "linear"),
-                                                   ("exp", None, "none"), ("matern3_2", "nugget", "constant")])
-def test_gradient_paths_agree(ctx, kernel, noise, regmodel):
-    """R^-1 by the backward block sweep (path 1) against L^-1 then L^-T L^-1
-    (path 0, default), in one uneven batch; the LL of a sweep evaluation also
-    equals the LL-only evaluation (which always forms L^-1)."""
-    sizes = [200, 333, 64, 517]
-    pids, gams = [], []
-    for i, n in enumerate(sizes):
-        X, y = _design(300 + i, n, 4, kernel)
-        pids.append(ctx.problem(X, y, O.trend_matrix(X, regmodel), kernel, noise))
-        th = [0.2 + 0.05 * i, 0.35, 0.5, 0.3]
-        gams.append(th + ([0.9] if noise == "nugget" else []))
-    gam = np.array(gams)
-    try:
-        assert ctx.gradient_path() == 0
-        ctx.set_gradient_path(1)
-        ll1, g1, st1, inc1 = ctx.ll_grad_batch(pids, gam, True)
-        ll_only, _, st_o, _ = ctx.ll_grad_batch(pids, gam, False)
-        ctx.set_gradient_path(0)
-        ll0, g0, st0, inc0 = ctx.ll_grad_batch(pids, gam, True)
-        for b in range(len(pids)):
-            assert st1[b] == 0 and st0[b] == 0 and st_o[b] == 0 and inc1[b] == inc0[b]
-            assert abs(ll1[b] - ll0[b]) <= 1e-11 * abs(ll0[b])
-            assert abs(ll1[b] - ll_only[b]) <= 1e-11 * abs(ll_only[b])
-            assert rel_err(g1[b], g0[b]) <= 1e-9, (b, rel_err(g1[b], g0[b]))
-    finally:
-        ctx.set_gradient_path(0)
-        for p in pids:
-            ctx.drop_problem(p)
-
-
-def test_rinv_path_jitter_ladder_and_golden(ctx):
-    """the R^-1 sweep path through the retry ladder (singular R: its rcond
-    bound fails and the exact check decides) and on the golden vectors"""
-    X, y = _design(9, 80, 2, "gauss")
-    X = np.vstack([X, X[:5]])
-    y = np.concatenate([y, y[:5]])
-    ctx.set_gradient_path(1)
-    try:
-        _check(ctx, X, y, "gauss", "constant", None, np.array([[0.3, 0.3]]), names=["duplicates"])
-        for prefix in ["data1-scal", "data2-grad-10"]:
-            Xg, yg, llr, gr = _load(prefix)
-            pid = ctx.problem(Xg, yg, np.ones((len(yg), 1)), "gauss")
-            try:
-                ll, g, st, _ = ctx.ll_grad_batch([pid], np.full((1, Xg.shape[1]), 0.3), True)
-                assert st[0] == 0
-                assert abs(ll[0] - llr) <= LL_TOL * abs(llr)
-                assert rel_err(g[0], gr) <= GRAD_TOL
-            finally:
-                ctx.drop_problem(pid)
-    finally:
-        ctx.set_gradient_path(0)
 
 
 def test_bucket_parts_match_one_stream(ctx):
