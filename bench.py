@@ -38,24 +38,62 @@ def workload(seed=SEED, n=N_POINTS, d=DIM, nstart=NSTART):
     return X, y, theta
 
 
-def cpu_baseline(X, y, theta, gpu_ll, gpu_grad, budget_s=15.0, max_evals=20):
-    """The CPU oracle (a C restatement of the reference algorithm, 1 thread)
-    timed on a bounded sample of the same 64 evaluations; also returns the
-    parity of the GPU results on that sample."""
+def host_cpu():
+    """(model name, cores this process may use) of the host"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(X, y, theta, gpu_ll, gpu_grad, budget_s=10.0, max_evals=20):
+    """The reference algorithm on the host: the CPU oracle (a C restatement
+    with the reference's serial pair loops) calling LAPACK's dpotrf / dtrtrs /
+    dtrcon as the reference does through Armadillo (scipy's OpenBLAS build),
+    timed on a bounded sample of the same 64 evaluations, first with one
+    BLAS thread (the reference's multistart fit runs max(1, nproc / 64)
+    threads, Kriging.cpp:1744-1746) and then with every core of this
+    process; also the restated-loop oracle of the parity tests, and the
+    parity of the GPU results on the sample."""
+    from threadpoolctl import threadpool_limits
     from oracle import oracle as O
+    model, ncores = host_cpu()
     P = O.OracleProblem(X, y, "exp", "constant")
-    t0 = time.perf_counter()
-    k = 0
-    ll_err, g_err = 0.0, 0.0
-    while k < max_evals and (time.perf_counter() - t0) < budget_s:
-        ll, g, _ = P.log_likelihood(theta[k])
-        ll_err = max(ll_err, abs(gpu_ll[k] - ll) / abs(ll))
-        g_err = max(g_err, np.linalg.norm(gpu_grad[k] - g) / np.linalg.norm(g))
-        k += 1
-    dt = time.perf_counter() - t0
-    return {"value": k / dt, "unit": "evals/s", "cores": 1, "kind": "port",
-            "sample": f"{k} of the {theta.shape[0]} LL+grad evaluations (n={X.shape[0]}, d={X.shape[1]}), "
-                      f"single-threaded C oracle"}, ll_err, g_err
+
+    def timed(threads, lapack, limit):
+        O.use_lapack(lapack)
+        ll_err, g_err = 0.0, 0.0
+        k = 0
+        with threadpool_limits(threads):
+            t0 = time.perf_counter()
+            while k < limit and (time.perf_counter() - t0) < budget_s:
+                ll, g, _ = P.log_likelihood(theta[k])
+                ll_err = max(ll_err, abs(gpu_ll[k] - ll) / abs(ll))
+                g_err = max(g_err, np.linalg.norm(gpu_grad[k] - g) / np.linalg.norm(g))
+                k += 1
+            dt = time.perf_counter() - t0
+        O.use_lapack(False)
+        return k / dt, k, ll_err, g_err
+
+    try:
+        v1, k1, e1, g1 = timed(1, True, max_evals)
+        va, ka, ea, ga = timed(ncores, True, max_evals)
+        vr, kr, er, gr = timed(1, False, 5)
+    finally:
+        O.use_lapack(False)
+    sample = (f"{k1} of the {theta.shape[0]} LL+grad evaluations (n={X.shape[0]}, d={X.shape[1]}): C oracle "
+              f"with the reference's serial pair loops, LAPACK dpotrf/dtrtrs/dtrcon (scipy OpenBLAS), 1 BLAS thread")
+    cb = {"value": v1, "unit": "evals/s", "cores": 1, "kind": "port", "sample": sample,
+          "cpu_model": model, "host_cores": ncores,
+          "all_cores": {"value": va, "cores": ncores, "evaluations": ka},
+          "restated_loops_1_thread": {"value": vr, "evaluations": kr}}
+    return cb, max(e1, ea, er), max(g1, ga, gr)
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_batch64_n1024_d6.txt")
@@ -74,21 +112,24 @@ def lauum_traffic_bytes():
     return None
 
 
-def mfma_probe_sweep(N):
-    """FP64 MFMA issue rate from registers (lkgpu_mfma_probe) over independent
-    chains per wave and workgroups (256 threads) per launch"""
-    ctx = N.Context(0)
-    print(f"{'chains':>6s} {'workgroups':>10s} {'ms':>9s} {'TFLOP/s':>8s}")
+def mfma_probe_sweep():
+    """FP64 MFMA issue rate from registers (the probe library) over
+    independent chains per wave and workgroups (256 threads) per launch, with
+    the mean shader clock of the workgroups"""
+    from libkriging_amd import probe
+    print(f"{'chains':>6s} {'workgroups':>10s} {'ms':>9s} {'TFLOP/s':>8s} {'clock MHz':>9s}")
     for chains in (1, 2, 4, 8):
         for blocks in (256, 512, 1024, 2048):
-            ms, tf = ctx.mfma_probe(blocks, 20000 if blocks <= 1024 else 10000, chains)
-            print(f"{chains:6d} {blocks:10d} {ms:9.3f} {tf:8.2f}", flush=True)
+            ms, tf, mhz = probe.mfma(blocks, 20000 if blocks <= 1024 else 10000, chains)
+            print(f"{chains:6d} {blocks:10d} {ms:9.3f} {tf:8.2f} {mhz:9.0f}", flush=True)
 
 
 def cholesky_large(N, n=8192, d=10, reps=3):
-    """The single-evaluation Cholesky at n = 8192 (gauss kernel, d = 10,
-    theta log-U[0.3, 1.5], nugget ratio 1e-6), timed live over its own
-    launches: n^3 / 3 flops per factorisation against the FP64 MFMA peak."""
+    """The single-evaluation configuration at n = 8192 (gauss kernel, d = 10,
+    theta log-U[0.3, 1.5], nugget ratio 1e-6): the Cholesky timed live over
+    its own launches (n^3 / 3 flops per factorisation against the FP64 MFMA
+    peak), the L^-1 stage likewise (n^3 / 3 flops), and the whole LL +
+    gradient (11 components) by wall clock."""
     rng = np.random.default_rng(125)
     X = rng.uniform(size=(n, d))
     y = rng.normal(size=n)
@@ -101,12 +142,26 @@ def cholesky_large(N, n=8192, d=10, reps=3):
     for _ in range(reps):
         _, _, st, _ = ctx.ll_grad_batch([pid], gam, False)
     ms, count = ctx.take_stage_time()
+    ctx.set_timed_stage("trtri")
+    for _ in range(reps):
+        ctx.ll_grad_batch([pid], gam, True)
+    ms_tri, _ = ctx.take_stage_time()
     ctx.set_timed_stage(-1)
     assert (st == 0).all()
+    ctx.ll_grad_batch([pid], gam, True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.ll_grad_batch([pid], gam, True)
+    ctx.synchronize()
+    ms_llg = (time.perf_counter() - t0) / reps * 1e3
     tflops = n ** 3 / 3.0 / (ms * 1e-3) / 1e12
+    tri_tf = n ** 3 / 3.0 / (ms_tri * 1e-3) / 1e12
     ctx.drop_problem(pid)
     return {"n": n, "avg_ms": ms, "runs": count, "achieved_tflops": tflops,
-            "frac_of_fp64_mfma_peak": tflops / MFMA_FP64_PEAK_TFLOPS}
+            "frac_of_fp64_mfma_peak": tflops / MFMA_FP64_PEAK_TFLOPS,
+            "trtri_ms": ms_tri, "trtri_frac_of_fp64_mfma_peak": tri_tf / MFMA_FP64_PEAK_TFLOPS,
+            "ll_grad_ms": ms_llg}
 
 
 def nested_workload(seed=127, n=200_000, d=5, groups=256):
@@ -134,42 +189,55 @@ def nested_workload(seed=127, n=200_000, d=5, groups=256):
     return X, y, group_of, sizes, Xn
 
 
-def run_nested(args):
-    """One NestedKriging fit (all 256 groups' BFGS in shared batched
-    pipelines) and one NK predict; value = predicted points per second."""
+def nested_leg(points=2000):
+    """The uneven NestedKriging configuration: fit (all 256 groups' BFGS in
+    shared batched pipelines) and NK predict on `points` of the 10 000
+    prediction points, the pair-contraction kernel timed live with HIP
+    events: 2 sum_{i<j} n_i n_j flops per point."""
     import libkriging_amd as lk
     X, y, group_of, sizes, Xn = nested_workload()
-    q = args.nested_points
-    Xq = Xn[:q]
+    Xq = Xn[:points]
     nk = lk.NestedKriging("matern3_2")
     t0 = time.perf_counter()
     nk.fit_groups(y, X, group_of, "constant", "BFGS")
     t_fit = time.perf_counter() - t0
     fit_evals, fit_batches = nk.fit_statistics()
     nk.predict(Xq[:64], True)  # warm
+    nk.time_pairs(True)
     t0 = time.perf_counter()
     mean, sd = nk.predict(Xq, True)
     t_pred = time.perf_counter() - t0
+    pair_ms, pair_launches, pair_points = nk.pair_time()
+    nk.time_pairs(False)
     s = sizes.astype(np.float64)
-    pair_flops = (s.sum() ** 2 - np.sum(s ** 2)) * q  # 2 sum_{i<j} n_i n_j per point
-    achieved = pair_flops / t_pred / 1e12
-    line = {
-        "metric": "NestedKriging NK predict points/s (N=200000, 256 uneven groups, d=5, matern3_2)",
-        "value": q / t_pred, "unit": "points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
-        "ms_per_step": 1e3 * t_pred, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "NestedKriging uneven groups: fit (BFGS per group) + NK predict",
-                   "n": int(s.sum()), "d": 5, "groups": len(sizes), "group_min": int(sizes.min()),
-                   "group_max": int(sizes.max()), "points": q, "kernel": "matern3_2"},
+    pair_flops = (s.sum() ** 2 - np.sum(s ** 2)) * pair_points
+    achieved = pair_flops / (pair_ms * 1e-3) / 1e12
+    return {
+        "config": {"n": int(s.sum()), "d": 5, "groups": len(sizes), "group_min": int(sizes.min()),
+                   "group_max": int(sizes.max()), "points": points, "kernel": "matern3_2"},
         "fit_s": t_fit, "fit_evaluations": int(fit_evals), "fit_batches": int(fit_batches),
         "fit_evals_per_s": fit_evals / t_fit,
-        "predict_fallbacks": nk.predict_fallbacks(),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / MFMA_FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_nk_pair",
-                     "note": "pair-contraction flops over the whole predict call (wall clock)"},
+        "nk_points_per_s": points / t_pred, "predict_s": t_pred, "predict_fallbacks": nk.predict_fallbacks(),
+        "nk_pair": {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / MFMA_FP64_PEAK_TFLOPS, "kernel": "k_nk_pair", "ms": pair_ms,
+                    "launches": pair_launches},
         "theta": nk.theta().tolist(), "sigma2": nk.sigma2(), "beta0": nk.beta0(),
         "finite": bool(np.all(np.isfinite(mean)) and np.all(np.isfinite(sd))),
     }
+
+
+def run_nested(args):
+    """The nested configuration on its own line: value = NK points/s."""
+    r = nested_leg(args.nested_points)
+    line = {
+        "metric": "NestedKriging NK predict points/s (N=200000, 256 uneven groups, d=5, matern3_2)",
+        "value": r["nk_points_per_s"], "unit": "points/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * r["predict_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": dict(r["config"], workload="NestedKriging uneven groups: fit (BFGS per group) + NK predict"),
+        "roofline": r["nk_pair"],
+    }
+    line.update({k: v for k, v in r.items() if k not in ("config", "nk_pair")})
     print(json.dumps(line))
 
 
@@ -181,6 +249,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-chol", action="store_true", help="skip the n = 8192 Cholesky measurement")
+    ap.add_argument("--no-nested", action="store_true", help="skip the NestedKriging fit + NK predict leg")
     ap.add_argument("--workload", default="multistart", choices=["multistart", "nested"],
                     help="multistart: the headline batched LL+grad; nested: NestedKriging fit + NK predict")
     ap.add_argument("--nested-points", type=int, default=2000, help="prediction points of the nested workload")
@@ -196,8 +265,7 @@ def main():
         sys.exit(2)
 
     if args.mfma_probe_sweep:
-        from libkriging_amd import _native as N
-        mfma_probe_sweep(N)
+        mfma_probe_sweep()
         return
     if args.workload == "nested":
         run_nested(args)
@@ -294,14 +362,19 @@ def main():
         "split_equals_single_stream": split_equals_single,
         "roofline": roof,
     }
-    copy_ms = ctx.device_copy(COPY_BYTES, 5)
+    from libkriging_amd import probe
+    copy_ms = probe.device_copy(COPY_BYTES, 5)
     line["hbm_copy_GBps"] = 2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9  # read + write of a plain copy kernel
     # FP64 MFMA rate reachable from registers alone (8 independent
     # accumulation chains per wave, 8 waves per SIMD): the practical ceiling
-    # beside the published 78.6 TFLOP/s
-    line["mfma_f64_probe_TFLOPs"] = ctx.mfma_probe(2048, 10000, 8)[1]
+    # beside the published 78.6 TFLOP/s, and the shader clock it ran at
+    _, tf, mhz = probe.mfma(2048, 10000, 8)
+    line["mfma_f64_probe_TFLOPs"] = tf
+    line["mfma_f64_probe_clock_MHz"] = mhz
     if not args.no_chol:
         line["cholesky_n8192"] = cholesky_large(N)
+    if not args.no_nested:
+        line["nested_c5"] = nested_leg(args.nested_points)
     if not args.no_cpu:
         cb, ll_err, g_err = cpu_baseline(X, y, theta, ll, grad)
         line["cpu_baseline"] = cb
@@ -311,7 +384,8 @@ def main():
         # a plain device copy of known size, run in the same process so that
         # a counter-collection pass over this command also measures it:
         # FETCH_SIZE / WRITE_SIZE of this kernel calibrate those counters
-        ms = ctx.device_copy(COPY_BYTES, 3)  # k_copy: reads and writes COPY_BYTES per launch
+        from libkriging_amd import probe
+        ms = probe.device_copy(COPY_BYTES, 3)  # k_copy: reads and writes COPY_BYTES per launch
         print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "copy_ms": ms,
                           "copy_GBps": 2 * COPY_BYTES / (ms * 1e-3) / 1e9}), file=sys.stderr)
 

@@ -176,18 +176,6 @@ int lkgpu_set_streams(lkgpu_context* ctx, int streams);
 int lkgpu_set_tuning(lkgpu_context* ctx, const char* json_text);
 int lkgpu_last_parts(lkgpu_context* ctx);
 
-/* A plain device-to-device copy kernel over `bytes` (a multiple of 16),
- * repeated `reps` times; *ms_out = mean time per copy.  No counterpart in the
- * reference: the HBM bandwidth reference and the calibration of the
- * FETCH_SIZE / WRITE_SIZE counters behind the roofline traffic figures. */
-int lkgpu_device_copy(lkgpu_context* ctx, size_t bytes, int reps, double* ms_out);
-
-/* FP64 MFMA issue-rate probe: `blocks` workgroups of 4 waves, each wave
- * running `iters` rounds of `chains` (1, 2, 4 or 8) independent
- * 16x16x4 FP64 MFMA accumulations from registers; *ms_out = kernel time,
- * *tflops_out = achieved FP64 TFLOP/s.  No counterpart in the reference: the
- * measured ceiling beside the published 78.6 TFLOP/s peak. */
-int lkgpu_mfma_probe(lkgpu_context* ctx, int blocks, int iters, int chains, double* ms_out, double* tflops_out);
 
 /* ------------------------------------------------------------------------ */
 /* Layer 2: the Kriging object (libkriging_c.h names and conventions)       */
@@ -368,6 +356,12 @@ int lk_nested_kriging_fit_statistics(void* ptr, int* evaluations, int* batches);
  * nb_groups each (beta0 0 without a constant trend); any may be NULL.  No
  * reference counterpart: the input of the unification, for checking it. */
 int lk_nested_kriging_group_fits(void* ptr, double* theta_out, double* sigma2_out, double* beta0_out);
+/* HIP-event timing of the NK pair-contraction launches (k_nk_pair) of the
+ * following predicts (no reference counterpart: the kernel's roofline
+ * figure); lk_nested_kriging_pair_time returns the milliseconds, launches
+ * and points accumulated since the previous call and resets them. */
+int lk_nested_kriging_time_pairs(void* ptr, int enable);
+int lk_nested_kriging_pair_time(void* ptr, double* ms_out, int* launches_out, int* points_out);
 
 #ifdef __cplusplus
 }

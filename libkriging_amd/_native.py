@@ -23,9 +23,6 @@ NOISES = {None: 0, "none": 0, "nugget": 1}
 # exported symbols and their (restype, argtypes); every one is declared in
 # include/lkgpu.h
 SIGNATURES = {
-    "lkgpu_device_copy": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
-    "lkgpu_mfma_probe": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
-                                        ctypes.POINTER(ctypes.c_double)]),
     "lk_get_last_error": (ctypes.c_char_p, []),
     "lkgpu_build_id": (ctypes.c_char_p, []),
     "lkgpu_context_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -195,19 +192,6 @@ class Context:
 
     def set_timed_stage(self, stage):
         check(lib().lkgpu_set_timed_stage(self.h, STAGES[stage] if isinstance(stage, str) else int(stage)))
-
-    def device_copy(self, nbytes, reps=3):
-        """mean ms of a plain device copy kernel over nbytes"""
-        ms = ctypes.c_double()
-        check(lib().lkgpu_device_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
-        return ms.value
-
-    def mfma_probe(self, blocks, iters, chains=4):
-        """(ms, TFLOP/s) of the FP64 MFMA issue-rate probe"""
-        ms, tf = ctypes.c_double(), ctypes.c_double()
-        check(lib().lkgpu_mfma_probe(self.h, int(blocks), int(iters), int(chains), ctypes.byref(ms),
-                                     ctypes.byref(tf)))
-        return ms.value, tf.value
 
     def take_stage_time(self):
         ms = ctypes.c_double()

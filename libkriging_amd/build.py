@@ -55,6 +55,23 @@ def _compile(src, defines=()):
     return obj
 
 
+PROBE_SRC = os.path.join(CSRC, "probe", "lk_probe.hip")
+PROBE_LIB = os.path.join(HERE, "liblkprobe.so")
+
+
+def build_probe(verbose=True):
+    """liblkprobe.so: the benchmark's copy and MFMA probes (not the library)"""
+    tmp = PROBE_LIB + ".tmp"
+    cmd = [HIPCC, "-x", "hip", *COMMON, "-shared", PROBE_SRC, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"probe build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, PROBE_LIB)
+    if verbose:
+        print(f"built {PROBE_LIB}")
+    return PROBE_LIB
+
+
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -71,6 +88,7 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_probe(verbose)
     return LIB
 
 

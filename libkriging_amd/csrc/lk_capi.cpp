@@ -172,77 +172,6 @@ int lkgpu_set_tuning(lkgpu_context* ctx, const char* json_text) {
 int lkgpu_last_parts(lkgpu_context* ctx) { return ctx ? ctx->rt.last_parts() : -1; }
 
 
-int lkgpu_device_copy(lkgpu_context* ctx, size_t bytes, int reps, double* ms_out) {
-  return guarded([&] {
-    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
-    if (bytes == 0 || bytes % 16 != 0 || reps <= 0) throw std::invalid_argument("bytes must be a positive multiple of 16");
-    lk::hip_check(hipSetDevice(ctx->rt.device()), "hipSetDevice");
-    void *src = nullptr, *dst = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    auto cleanup = [&] {
-      if (src) hipFree(src);
-      if (dst) hipFree(dst);
-      if (e0) hipEventDestroy(e0);
-      if (e1) hipEventDestroy(e1);
-    };
-    try {
-      lk::hip_check(hipMalloc(&src, bytes), "hipMalloc");
-      lk::hip_check(hipMalloc(&dst, bytes), "hipMalloc");
-      lk::hip_check(hipEventCreate(&e0), "event");
-      lk::hip_check(hipEventCreate(&e1), "event");
-      hipStream_t st = ctx->rt.stream();
-      lk::hip_check(lk::launch_copy(src, dst, bytes, st), "copy");  // warm
-      lk::hip_check(hipEventRecord(e0, st), "event");
-      for (int r = 0; r < reps; ++r) lk::hip_check(lk::launch_copy(dst, src, bytes, st), "copy");
-      lk::hip_check(hipEventRecord(e1, st), "event");
-      lk::hip_check(hipEventSynchronize(e1), "event sync");
-      float ms = 0.f;
-      lk::hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
-      if (ms_out) *ms_out = ms / reps;
-    } catch (...) {
-      cleanup();
-      throw;
-    }
-    cleanup();
-  });
-}
-
-int lkgpu_mfma_probe(lkgpu_context* ctx, int blocks, int iters, int chains, double* ms_out, double* tflops_out) {
-  return guarded([&] {
-    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
-    if (blocks <= 0 || iters <= 0 || !(chains == 1 || chains == 2 || chains == 4 || chains == 8))
-      throw std::invalid_argument("blocks, iters > 0 and chains in {1, 2, 4, 8}");
-    lk::hip_check(hipSetDevice(ctx->rt.device()), "hipSetDevice");
-    double* out = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    auto cleanup = [&] {
-      if (out) hipFree(out);
-      if (e0) hipEventDestroy(e0);
-      if (e1) hipEventDestroy(e1);
-    };
-    try {
-      lk::hip_check(hipMalloc(&out, sizeof(double) * (size_t)blocks * 256), "hipMalloc");
-      lk::hip_check(hipEventCreate(&e0), "event");
-      lk::hip_check(hipEventCreate(&e1), "event");
-      hipStream_t st = ctx->rt.stream();
-      lk::hip_check(lk::launch_mfma_probe(blocks, iters, chains, out, st), "mfma probe");  // warm
-      lk::hip_check(hipEventRecord(e0, st), "event");
-      lk::hip_check(lk::launch_mfma_probe(blocks, iters, chains, out, st), "mfma probe");
-      lk::hip_check(hipEventRecord(e1, st), "event");
-      lk::hip_check(hipEventSynchronize(e1), "event sync");
-      float ms = 0.f;
-      lk::hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
-      const double flop = (double)blocks * 4.0 * iters * chains * 2.0 * 16 * 16 * 4;
-      if (ms_out) *ms_out = ms;
-      if (tflops_out) *tflops_out = flop / (ms * 1e-3) / 1e12;
-    } catch (...) {
-      cleanup();
-      throw;
-    }
-    cleanup();
-  });
-}
-
 int lkgpu_set_timed_stage(lkgpu_context* ctx, int stage) {
   return guarded([&] {
     if (ctx == nullptr) throw std::invalid_argument("context is NULL");

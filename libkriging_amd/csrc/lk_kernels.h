@@ -176,7 +176,6 @@ struct PredictArgs {
 hipError_t launch_predict(const PredictArgs& a, hipStream_t stream);
 hipError_t launch_predict_cov(const PredictArgs& a, hipStream_t stream);    // after launch_predict (same chunk)
 hipError_t launch_predict_deriv(const PredictArgs& a, hipStream_t stream);  // after launch_predict (same chunk)
-// plain device copy of `bytes` (a multiple of 16)
 // log-marginal-posterior gradient: the fused R^-1 / adjoint pass with
 // B = R^-1 - G'G in place of R^-1 (after launch_loo_g)
 hipError_t launch_lmp_adjoint(const Batch& b);
@@ -190,9 +189,6 @@ hipError_t launch_loo_adjoint(const Batch& b);
 hipError_t launch_loo_finalize_grad(const Batch& b);
 // R^-1 = L^-T L^-1 in full into A (after an L^-1-path evaluation)
 hipError_t launch_rinv_store(const Batch& b);
-hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream);
-// chains in {1, 2, 4, 8}; out: blocks * 256 doubles
-hipError_t launch_mfma_probe(int blocks, int iters, int chains, double* out, hipStream_t stream);
 
 // ---------------------------------------------------------------------------
 // NestedKriging "NK" aggregation (NestedKriging::predict_nk,

@@ -1183,49 +1183,6 @@ __global__ __launch_bounds__(TPB) void k_predict_reduce(PredictArgs a) {
   a.out[a.m + j] = v;
 }
 
-// plain device copy (16-byte vectors, grid-stride): the HBM bandwidth and
-// traffic-counter reference for the roofline figures
-__global__ __launch_bounds__(TPB) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) dst[i] = src[i];
-}
-
-hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t stream) {
-  const size_t n = bytes / 16;
-  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(TPB), 0, stream, (const double2*)src, (double2*)dst, n);
-  return hipGetLastError();
-}
-
-// FP64 MFMA issue-rate probe: every wave runs `iters` rounds of CH
-// independent V_MFMA_F64_16X16X4_F64 accumulation chains on register
-// operands (no memory traffic in the loop); each thread stores one sum so the
-// chains stay live.  out holds gridDim.x * TPB doubles.
-template <int CH>
-__global__ __launch_bounds__(TPB) void k_mfma_probe(int iters, double* __restrict__ out) {
-  d4 acc[CH];
-#pragma unroll
-  for (int c = 0; c < CH; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
-  const double a = 1.0 + 1e-12 * threadIdx.x, b = 1.0 - 1e-12 * threadIdx.x;
-  for (int it = 0; it < iters; ++it) {
-#pragma unroll
-    for (int c = 0; c < CH; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[c], 0, 0, 0);
-  }
-  double s = 0.0;
-#pragma unroll
-  for (int c = 0; c < CH; ++c) s += acc[c][0] + acc[c][1] + acc[c][2] + acc[c][3];
-  out[(size_t)blockIdx.x * TPB + threadIdx.x] = s;
-}
-
-hipError_t launch_mfma_probe(int blocks, int iters, int chains, double* out, hipStream_t stream) {
-  switch (chains) {
-    case 1: hipLaunchKernelGGL(k_mfma_probe<1>, dim3(blocks), dim3(TPB), 0, stream, iters, out); break;
-    case 2: hipLaunchKernelGGL(k_mfma_probe<2>, dim3(blocks), dim3(TPB), 0, stream, iters, out); break;
-    case 4: hipLaunchKernelGGL(k_mfma_probe<4>, dim3(blocks), dim3(TPB), 0, stream, iters, out); break;
-    case 8: hipLaunchKernelGGL(k_mfma_probe<8>, dim3(blocks), dim3(TPB), 0, stream, iters, out); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
 // ---------------------------------------------------------------------------
 // (10) predictive covariance: Sigma = R_nn - C' C + Ecirc Ecirc' (before
 // var_scale), C = L^-1 R_on.  R_nn as LinearAlgebra::covMat_sym_X

@@ -103,7 +103,28 @@ NestedKriging::NestedKriging(const std::string& kernel) : kernel_(kernel) {
   device_ = default_device();
 }
 
-NestedKriging::~NestedKriging() { release_nk(); }
+NestedKriging::~NestedKriging() {
+  release_nk();
+  for (auto& e : pev_)
+    if (e) hipEventDestroy(e);
+}
+
+void NestedKriging::set_pair_timing(bool on) {
+  if (on)
+    for (auto& e : pev_)
+      if (e == nullptr) hip_check(hipEventCreate(&e), "hipEventCreate");
+  time_pairs_ = on;
+  pair_ms_ = 0.0;
+  pair_launches_ = pair_points_ = 0;
+}
+
+void NestedKriging::take_pair_time(double* ms, int* launches, int* points) {
+  if (ms) *ms = pair_ms_;
+  if (launches) *launches = pair_launches_;
+  if (points) *points = pair_points_;
+  pair_ms_ = 0.0;
+  pair_launches_ = pair_points_ = 0;
+}
 
 void NestedKriging::release_nk() {
   if (rt_ != nullptr) hipSetDevice(device_);
@@ -568,11 +589,20 @@ void NestedKriging::predict_nk(const double* Xn, int m, bool want_stdev, double*
       a.kernel = kernel_id(kernel_);
       a.npairs = npairs_;
       hip_check(launch_nk_prepare(a, st), "nk prepare");
+      if (time_pairs_) hip_check(hipEventRecord(pev_[0], st), "event record");
       hip_check(launch_nk_pairs(a, st), "nk pairs");
+      if (time_pairs_) hip_check(hipEventRecord(pev_[1], st), "event record");
       hip_check(launch_nk_solve(a, st), "nk solve");
       hip_check(hipMemcpyAsync(hout.data(), dout, sizeof(double) * 2 * mc, hipMemcpyDeviceToHost, st), "download");
       hip_check(hipMemcpyAsync(hinfo.data(), dinfo, sizeof(int) * mc, hipMemcpyDeviceToHost, st), "download");
       hip_check(hipStreamSynchronize(st), "sync");
+      if (time_pairs_) {
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, pev_[0], pev_[1]), "event elapsed");
+        pair_ms_ += ms;
+        pair_points_ += mc;
+        ++pair_launches_;
+      }
       for (int t = 0; t < mc; ++t) {
         double mu = hout[t], var = hout[mc + t];
         if (hinfo[t] != 0) ++fallbacks_;

@@ -60,6 +60,10 @@ class NestedKriging {
   int n() const { return n_; }
   int d() const { return d_; }
   int predict_fallbacks() const { return fallbacks_; }  // points solved by LU / pinv in the last predict
+  // HIP-event timing of the NK pair-contraction launches of later predicts
+  void set_pair_timing(bool on);
+  // (ms, launches, points) accumulated since the last call; resets them
+  void take_pair_time(double* ms, int* launches, int* points);
   // evaluations and batched pipelines of the groups' optimisation in the last fit
   int fit_evaluations() const { return fit_evals_; }
   // the groups' own fitted hyper-parameters, before unification: theta
@@ -98,6 +102,10 @@ class NestedKriging {
   bool fitted_ = false;
   int fallbacks_ = 0;
   int fit_evals_ = 0, fit_batches_ = 0;
+  bool time_pairs_ = false;
+  hipEvent_t pev_[2] = {nullptr, nullptr};
+  double pair_ms_ = 0.0;
+  int pair_launches_ = 0, pair_points_ = 0;
 
   // NK device state (group-ordered, padded to 64-row tiles)
   int device_ = 0;

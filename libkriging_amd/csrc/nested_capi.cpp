@@ -238,6 +238,14 @@ int lk_nested_kriging_group_fits(void* ptr, double* theta_out, double* sigma2_ou
   });
 }
 
+int lk_nested_kriging_time_pairs(void* ptr, int enable) {
+  return guarded([&] { NK(ptr)->set_pair_timing(enable != 0); });
+}
+
+int lk_nested_kriging_pair_time(void* ptr, double* ms_out, int* launches_out, int* points_out) {
+  return guarded([&] { NK(ptr)->take_pair_time(ms_out, launches_out, points_out); });
+}
+
 int lk_nested_kriging_predict_fallbacks(void* ptr) {
   int v = -1;
   guarded([&] { v = NK(ptr)->predict_fallbacks(); });

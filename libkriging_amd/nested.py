@@ -42,6 +42,8 @@ N.register("lk_nested_kriging_set_predict_chunk", _i, [_vp, _i])
 N.register("lk_nested_kriging_predict_fallbacks", _i, [_vp])
 N.register("lk_nested_kriging_fit_statistics", _i, [_vp, _ip, _ip])
 N.register("lk_nested_kriging_group_fits", _i, [_vp, _dp, _dp, _dp])
+N.register("lk_nested_kriging_time_pairs", _i, [_vp, _i])
+N.register("lk_nested_kriging_pair_time", _i, [_vp, _dp, _ip, _ip])
 
 
 def _nparams(parameters, d):
@@ -212,6 +214,16 @@ class NestedKriging:
         th, s2, b0 = np.empty((p, d)), np.empty(p), np.empty(p)
         N.check(N.lib().lk_nested_kriging_group_fits(self._h, N.ptr(th), N.ptr(s2), N.ptr(b0)))
         return th, s2, b0
+
+    def time_pairs(self, enable=True):
+        """HIP-event timing of the pair-contraction launches of later predicts"""
+        N.check(N.lib().lk_nested_kriging_time_pairs(self._h, int(bool(enable))))
+
+    def pair_time(self):
+        """(ms, launches, points) of the timed pair launches since the last call"""
+        ms, nl, npt = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        N.check(N.lib().lk_nested_kriging_pair_time(self._h, ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(npt)))
+        return ms.value, nl.value, npt.value
 
     def predict_fallbacks(self):
         return N.lib().lk_nested_kriging_predict_fallbacks(self._h)

@@ -150,10 +150,38 @@ void lko_corr_mat(int kernel, const double* X1, int n1, const double* X2, int n2
   if (dxp != dx) free(dxp);
 }
 
+/* Optional LAPACK routines: the ones the reference reaches through
+ * Armadillo (arma::chol -> dpotrf, arma::solve(trimatl / trimatu) ->
+ * dtrtrs, arma::rcond -> dtrcon).  When lko_set_lapack has set them, the
+ * factorisation, the triangular solves and the condition estimate call them
+ * (the pair loops stay serial C, as in the reference); NULL restores the
+ * restated loops.  The CPU baseline times the oracle this way. */
+typedef void (*lko_dpotrf_t)(char*, int*, double*, int*, int*);
+typedef void (*lko_dtrtrs_t)(char*, char*, char*, int*, int*, double*, int*, double*, int*, int*);
+typedef void (*lko_dtrcon_t)(char*, char*, char*, int*, double*, int*, double*, double*, int*, int*);
+static lko_dpotrf_t g_dpotrf = NULL;
+static lko_dtrtrs_t g_dtrtrs = NULL;
+static lko_dtrcon_t g_dtrcon = NULL;
+
+void lko_set_lapack(void* dpotrf, void* dtrtrs, void* dtrcon) {
+  g_dpotrf = (lko_dpotrf_t)dpotrf;
+  g_dtrtrs = (lko_dtrtrs_t)dtrtrs;
+  g_dtrcon = (lko_dtrcon_t)dtrcon;
+}
+
 /* Lower Cholesky in place (LAPACK dpotrf semantics: returns j+1 when the
  * j-th pivot is <= 0 or NaN, 0 on success; the strict upper triangle is set to
  * zero as arma::chol(L, X, "lower") returns a triangular matrix). */
 int lko_potrf_lower(double* A, int n) {
+  if (g_dpotrf) {
+    char uplo = 'L';
+    int info = 0, nn = n;
+    g_dpotrf(&uplo, &nn, A, &nn, &info);
+    if (info != 0) return info;
+    for (int j = 1; j < n; j++)
+      for (int i = 0; i < j; i++) A[i + (size_t)j * n] = 0.0;
+    return 0;
+  }
   for (int j = 0; j < n; j++) {
     double* cj = A + (size_t)j * n;
     for (int k = 0; k < j; k++) {
@@ -175,6 +203,12 @@ int lko_potrf_lower(double* A, int n) {
 
 /* B <- L^-1 B (forward substitution, L lower n x n, B n x nrhs). */
 void lko_solve_lower(const double* L, int n, double* B, int nrhs) {
+  if (g_dtrtrs) {
+    char uplo = 'L', trans = 'N', diag = 'N';
+    int nn = n, nr = nrhs, info = 0;
+    if (nr > 0) g_dtrtrs(&uplo, &trans, &diag, &nn, &nr, (double*)L, &nn, B, &nn, &info);
+    return;
+  }
   for (int c = 0; c < nrhs; c++) {
     double* b = B + (size_t)c * n;
     for (int j = 0; j < n; j++) {
@@ -189,6 +223,12 @@ void lko_solve_lower(const double* L, int n, double* B, int nrhs) {
 
 /* B <- L^-T B (back substitution with the upper factor L^T). */
 void lko_solve_upper_t(const double* L, int n, double* B, int nrhs) {
+  if (g_dtrtrs) {
+    char uplo = 'L', trans = 'T', diag = 'N';
+    int nn = n, nr = nrhs, info = 0;
+    if (nr > 0) g_dtrtrs(&uplo, &trans, &diag, &nn, &nr, (double*)L, &nn, B, &nn, &info);
+    return;
+  }
   for (int c = 0; c < nrhs; c++) {
     double* b = B + (size_t)c * n;
     for (int i = n - 1; i >= 0; i--) {
@@ -274,6 +314,17 @@ static double inv_norm1_estimate(const double* L, int n) {
 
 /* rcond of a lower triangular matrix in the 1-norm (dtrcon '1','L','N'). */
 double lko_trcon_lower(const double* L, int n) {
+  if (g_dtrcon && n > 0) {
+    char norm = '1', uplo = 'L', diag = 'N';
+    int nn = n, info = 0;
+    double rc = 0.0;
+    double* work = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+    int* iwork = (int*)malloc(sizeof(int) * (size_t)n);
+    g_dtrcon(&norm, &uplo, &diag, &nn, (double*)L, &nn, &rc, work, iwork, &info);
+    free(work);
+    free(iwork);
+    return rc;
+  }
   double anorm = 0.0;
   for (int j = 0; j < n; j++) {
     double s = 0.0;

@@ -94,6 +94,11 @@ int lko_predict(const lko_problem* P, const double* theta, double alpha, double 
 
 const char* lko_last_error(void);
 
+/* Route the factorisation, triangular solves and condition estimate through
+ * LAPACK's dpotrf / dtrtrs / dtrcon (Fortran calling convention without
+ * hidden string lengths); NULLs restore the restated loops. */
+void lko_set_lapack(void* dpotrf, void* dtrtrs, void* dtrcon);
+
 #ifdef __cplusplus
 }
 #endif

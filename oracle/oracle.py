@@ -89,6 +89,31 @@ def _f64(a):
     return np.asfortranarray(np.asarray(a, dtype=np.float64))
 
 
+def use_lapack(enable=True):
+    """Route the oracle's dpotrf / dtrtrs / dtrcon through scipy's LAPACK
+    (the routines the reference reaches through Armadillo) or back to the
+    restated loops.  BLAS threads are those of scipy's OpenBLAS
+    (threadpoolctl limits them)."""
+    L = lib()
+    L.lko_set_lapack.argtypes = [ctypes.c_void_p] * 3
+    L.lko_set_lapack.restype = None
+    if not enable:
+        L.lko_set_lapack(None, None, None)
+        return
+    import scipy.linalg.cython_lapack as cl
+    get = ctypes.pythonapi.PyCapsule_GetPointer
+    get.restype = ctypes.c_void_p
+    get.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    gname = ctypes.pythonapi.PyCapsule_GetName
+    gname.restype = ctypes.c_char_p
+    gname.argtypes = [ctypes.py_object]
+    ptrs = []
+    for name in ("dpotrf", "dtrtrs", "dtrcon"):
+        cap = cl.__pyx_capi__[name]
+        ptrs.append(get(cap, gname(cap)))
+    L.lko_set_lapack(*ptrs)
+
+
 def default_cfg(num_nugget=1e-10, rcond_check=True, max_inc=10, min_rcond=1e-18):
     return CholCfg(num_nugget, int(rcond_check), max_inc, min_rcond)
 

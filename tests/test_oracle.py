@@ -347,3 +347,32 @@ def test_noise_free_fixed_sigma2_matches_gaussian_density():
     P = O.OracleProblem(X, y, "exp", "linear", est_sigma2=False, sigma2=0.8)
     ll, _, _ = P.log_likelihood(theta, want_grad=False)
     assert rel_err(ll, _gauss_loglik(y, O.trend_matrix(X, "linear"), 0.8 * _numpy_corr(X, "exp", theta))) < 1e-11
+
+
+def test_lapack_route_matches_restated_loops():
+    """the oracle's optional LAPACK route (dpotrf / dtrtrs / dtrcon from
+    scipy, the CPU baseline's arithmetic) equals the restated loops on the
+    reference's golden vectors and on a nugget / linear-trend case"""
+    try:
+        for prefix in ["data1-scal", "data2-grad-7"]:
+            X, y, llr, gr = _load(prefix)
+            P = O.OracleProblem(X, y, "gauss")
+            O.use_lapack(True)
+            ll, g, inc = P.log_likelihood(0.3 * np.ones(X.shape[1]))
+            O.use_lapack(False)
+            assert rel_err(ll, llr) < 1e-12 and rel_err(g, gr) < 1e-12 and inc == 0
+        rng = np.random.default_rng(31)
+        X = rng.uniform(size=(120, 3))
+        y = np.sin(3 * X).sum(axis=1)
+        P = O.OracleProblem(X, y, "matern5_2", "linear", "nugget")
+        gam = np.array([0.3, 0.4, 0.5, 0.9])
+        a = P.log_likelihood(gam)
+        O.use_lapack(True)
+        b = P.log_likelihood(gam)
+        L, info = O.potrf_lower(P.build_R(gam[:3]) + np.eye(120))
+        rc = O.trcon_lower(np.tril(L))
+        O.use_lapack(False)
+        assert rel_err(a[0], b[0]) < 1e-13 and rel_err(a[1], b[1]) < 1e-12 and a[2] == b[2]
+        assert info == 0 and rel_err(rc, lapack.dtrcon(np.tril(L), norm="1", uplo="L")[0]) < 1e-12
+    finally:
+        O.use_lapack(False)
