@@ -409,6 +409,39 @@ __global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ sl
 // read and written 1 / P as often.
 // grid (tiles of the widest slot, nslots), 256 threads
 // ---------------------------------------------------------------------------
+// tile number `tile` of the update's tile set -> (r, c); false when out of range
+__device__ inline bool update_tile_rc(int tile, int nb, int c0, int c1, int& r, int& c) {
+  if (c1 < 0) {
+    const int T = nb - c0;
+    if (tile >= T * (T + 1) / 2) return false;
+    int ti, tj;
+    tri_decode(tile, ti, tj);
+    r = c0 + ti;
+    c = c0 + tj;
+    return true;
+  }
+  int t = tile;
+  c = c0;
+  const int cl = min(c1, nb - 1);
+  while (c <= cl && t >= nb - c) {
+    t -= nb - c;
+    ++c;
+  }
+  if (c > cl) return false;
+  r = c + t;
+  return true;
+}
+
+__device__ inline void update_tile(const Slot& S, int r, int c, int kb, int ke, double* sbuf) {
+  const int i0 = r * NB, j0 = c * NB, ld = S.npad;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  d4 acc[2][2];
+  double* C = S.A + (size_t)j0 * ld + i0;
+  acc_load(acc, C, ld, lane, wr, wc);
+  tile_mainloop<false, false, true>(S.A, ld, i0, S.A, ld, j0, kb * NB, ke * NB, sbuf, acc, tid, lane, wr, wc);
+  acc_store(acc, C, ld, lane, wr, wc);
+}
+
 __global__ __launch_bounds__(TPB) void k_potrf_update(const Slot* __restrict__ slots,
                                                       const SlotParams* __restrict__ params,
                                                       SlotOut* __restrict__ out, int kb, int ke, int c0, int c1) {
@@ -419,32 +452,31 @@ __global__ __launch_bounds__(TPB) void k_potrf_update(const Slot* __restrict__ s
   const int nb = S.nblk;
   if (c0 >= nb || ke > nb) return;
   int r, c;
-  if (c1 < 0) {
-    const int T = nb - c0;
-    if (ts.tile >= T * (T + 1) / 2) return;
-    int ti, tj;
-    tri_decode(ts.tile, ti, tj);
-    r = c0 + ti;
-    c = c0 + tj;
-  } else {
-    int t = ts.tile;
-    c = c0;
-    const int cl = min(c1, nb - 1);
-    while (c <= cl && t >= nb - c) {
-      t -= nb - c;
-      ++c;
-    }
-    if (c > cl) return;
-    r = c + t;
-  }
-  const int i0 = r * NB, j0 = c * NB, ld = S.npad;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  if (!update_tile_rc(ts.tile, nb, c0, c1, r, c)) return;
   __shared__ double sbuf[4 * KC * LDSS];
-  d4 acc[2][2];
-  double* C = S.A + (size_t)j0 * ld + i0;
-  acc_load(acc, C, ld, lane, wr, wc);
-  tile_mainloop<false, false, true>(S.A, ld, i0, S.A, ld, j0, kb * NB, ke * NB, sbuf, acc, tid, lane, wr, wc);
-  acc_store(acc, C, ld, lane, wr, wc);
+  update_tile(S, r, c, kb, ke, sbuf);
+}
+
+// The same update for one slot by a persistent grid: workgroup g takes tiles
+// g, g + G, g + 2G, ... (G = gridDim.x).  Launched with about two
+// workgroups per CU, it leaves room on every CU (LDS and registers) for the
+// diagonal-block, panel and look-ahead launches of the next panel, which
+// run beside it on another stream.  Each tile gets the same arithmetic as in
+// k_potrf_update.
+__global__ __launch_bounds__(TPB) void k_potrf_update_persistent(const Slot* __restrict__ slots,
+                                                                 const SlotParams* __restrict__ params,
+                                                                 SlotOut* __restrict__ out, int kb, int ke, int c0,
+                                                                 int c1, int tiles) {
+  const Slot S = slots[0];
+  if (!slot_live(params[0], out[0])) return;
+  const int nb = S.nblk;
+  if (c0 >= nb || ke > nb) return;
+  __shared__ double sbuf[4 * KC * LDSS];
+  for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+    int r, c;
+    if (!update_tile_rc(t, nb, c0, c1, r, c)) return;  // tiles are in increasing order: the rest are out too
+    update_tile(S, r, c, kb, ke, sbuf);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1432,6 +1464,22 @@ hipError_t launch_potrf_update(const Batch& B, int kb, int ke, int c0, int c1) {
   if (tiles <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_potrf_update, dim3(tiles, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, kb, ke,
                      c0, c1);
+  return hipGetLastError();
+}
+hipError_t launch_potrf_update_persistent(const Batch& B, int kb, int ke, int c0, int c1, int workgroups) {
+  if (B.nslots != 1) return hipErrorInvalidValue;
+  const int nb = B.nblk_max;
+  if (c0 >= nb) return hipSuccess;
+  int tiles = 0;
+  if (c1 < 0) {
+    tiles = tri(nb - c0);
+  } else {
+    for (int c = c0; c <= c1 && c < nb; ++c) tiles += nb - c;
+  }
+  if (tiles <= 0) return hipSuccess;
+  const int g = workgroups < 1 ? 1 : (workgroups < tiles ? workgroups : tiles);
+  hipLaunchKernelGGL(k_potrf_update_persistent, dim3(g), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, kb, ke,
+                     c0, c1, tiles);
   return hipGetLastError();
 }
 int trtri_levels(int nblk) {

@@ -35,6 +35,14 @@ Runtime::Runtime(int device) : device_(device) {
   if (device < 0 || device >= count) throw std::runtime_error("no HIP device " + std::to_string(device));
   hip_check(hipSetDevice(device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipStreamCreateWithFlags(&la_stream_, hipStreamNonBlocking), "hipStreamCreate");
+  for (auto& e : la_ev_) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  {
+    int cus = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
+    la_workgroups_ = 2 * std::max(1, cus);
+  }
+  if (const char* e = std::getenv("LKGPU_LOOKAHEAD")) lookahead_ = std::atoi(e) != 0;
   // the allocation-free entry point's scratch: its device address, looked
   // up once per device with this device current
   hip_check(noalloc_scratch(device, &scratch_), "hipGetSymbolAddress(scratch)");
@@ -144,6 +152,9 @@ Runtime::~Runtime() {
   if (h_params_) hipHostFree(h_params_);
   if (h_out_) hipHostFree(h_out_);
   if (fork_) hipEventDestroy(fork_);
+  for (auto& e : la_ev_)
+    if (e) hipEventDestroy(e);
+  if (la_stream_) hipStreamDestroy(la_stream_);
   for (int q = 0; q < kMaxStreams - 1; ++q) {
     if (join_[q]) hipEventDestroy(join_[q]);
     if (xstream_[q]) hipStreamDestroy(xstream_[q]);
@@ -332,7 +343,7 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
 // it belongs to (bit s = stage s): 0 build, 1 Cholesky, 2 first trailing
 // update, 3 trtri, 4 triangular solves + LL, 5 fused R^-1 / adjoint.
 std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_grad, int objective,
-                                                   int panel_blocks) const {
+                                                   int panel_blocks, bool lookahead) const {
   std::vector<Step> v;
   v.push_back({1u << 0, "build", [B] { return launch_build(B); }});
   // blocked right-looking Cholesky, two levels: panels of P block columns;
@@ -340,6 +351,44 @@ std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_gra
   // then one trailing update with K = 64 P per panel
   const int P = panel_blocks;
   bool first_update = true;
+  if (lookahead && B.nslots == 1 && B.nblk_max >= kLookaheadMinBlocks) {
+    // look-ahead: after panel p, its update of panel p + 1's columns runs on
+    // the main stream, then panel p + 1's chain (diagonal blocks, panels,
+    // in-panel updates) beside panel p's update of all later columns on the
+    // second stream.  Every tile still receives the panels' updates in panel
+    // order: the main stream waits for the bulk update of panel p - 1 before
+    // it applies panel p's update to panel p + 1.
+    Batch BL = B;
+    BL.stream = la_stream_;
+    const hipStream_t sa = B.stream, sl = la_stream_;
+    const hipEvent_t evA = la_ev_[0], evB = la_ev_[1];
+    const int wg = la_workgroups_;
+    bool pending = false;
+    for (int K0 = 0; K0 < B.nblk_max; K0 += P) {
+      const int K1 = std::min(K0 + P, B.nblk_max);
+      for (int k = K0; k < K1; ++k) {
+        v.push_back({1u << 1, "potrf_diag", [B, k] { return launch_potrf_diag(B, k); }});
+        v.push_back({1u << 1, "potrf_panel", [B, k] { return launch_potrf_panel(B, k); }});
+        if (k + 1 < K1)
+          v.push_back({1u << 1, "potrf_update (panel)", [B, k, K1] { return launch_potrf_update(B, k, k + 1, k + 1, K1 - 1); }});
+      }
+      if (K1 >= B.nblk_max) break;
+      const int K2 = std::min(K1 + P, B.nblk_max);
+      if (pending) v.push_back({1u << 1, "wait bulk update", [sa, evB] { return hipStreamWaitEvent(sa, evB, 0); }});
+      v.push_back({(1u << 1) | (first_update ? 1u << 2 : 0u), "potrf_update (next panel)",
+                   [B, K0, K1, K2] { return launch_potrf_update(B, K0, K1, K1, K2 - 1); }});
+      first_update = false;
+      if (K2 < B.nblk_max) {
+        v.push_back({1u << 1, "record chain", [sa, evA] { return hipEventRecord(evA, sa); }});
+        v.push_back({1u << 1, "fork bulk update", [sl, evA] { return hipStreamWaitEvent(sl, evA, 0); }});
+        v.push_back({1u << 1, "potrf_update (bulk, persistent)",
+                     [BL, K0, K1, K2, wg] { return launch_potrf_update_persistent(BL, K0, K1, K2, -1, wg); }});
+        v.push_back({1u << 1, "record bulk update", [sl, evB] { return hipEventRecord(evB, sl); }});
+        pending = true;
+      }
+    }
+    if (pending) v.push_back({1u << 1, "join bulk update", [sa, evB] { return hipStreamWaitEvent(sa, evB, 0); }});
+  } else
   for (int K0 = 0; K0 < B.nblk_max; K0 += P) {
     const int K1 = std::min(K0 + P, B.nblk_max);
     for (int k = K0; k < K1; ++k) {
@@ -512,7 +561,8 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
   hip_check(hipMemcpyAsync(d_params_, h_params_, sizeof(SlotParams) * nslots_, hipMemcpyHostToDevice, stream_),
             "upload params");
   const Part all{0, nslots_, nblk_max_, dmax_, tuning_[bucket_of(nblk_max_)].panel_blocks};
-  const std::vector<Step> steps = pipeline_steps(part_batch(all, stream_), want_grad, objective, all.panel_blocks);
+  const std::vector<Step> steps =
+      pipeline_steps(part_batch(all, stream_), want_grad, objective, all.panel_blocks, lookahead_ && nslots_ == 1);
   const auto rng = stage_range(steps, timed_stage_);
   const bool timed = rng.first < rng.second;
   const std::vector<Part> plan = plan_parts(timed, bench);

@@ -156,6 +156,11 @@ class Runtime {
   // load a table ({"buckets": [{"max_nblk", "panel_blocks", "streams"}, ...]})
   void load_tuning_json(const std::string& text);
   int bucket_of(int nblk) const;
+  // look-ahead Cholesky for one-slot batches of at least 16 blocks (off by
+  // default, LKGPU_LOOKAHEAD=1 turns it on): results are the same either
+  // way.  Measured slower at n = 8192 (9.3 vs 7.9 ms): the bulk update's
+  // workgroups keep the chain's panel launches from the CUs.
+  void set_lookahead(bool on) { lookahead_ = on; ++layout_version_; }
   // parts of the last run (slot ranges in bucket order) and their bucket
   int last_parts() const { return last_parts_; }
 
@@ -181,7 +186,8 @@ class Runtime {
   };
 
  private:
-  std::vector<Step> pipeline_steps(const Batch& B, bool want_grad, int objective, int panel_blocks) const;
+  std::vector<Step> pipeline_steps(const Batch& B, bool want_grad, int objective, int panel_blocks,
+                                   bool lookahead = false) const;
   // a contiguous slot range run as one unit of launches
   struct Part {
     int first = 0, count = 0, nblk_max = 0, dmax = 0, panel_blocks = 4;
@@ -201,6 +207,14 @@ class Runtime {
   std::vector<Part> buckets_;         // slot ranges of the buckets present (<= kMaxStreams)
   std::vector<BucketTuning> tuning_;  // ascending max_nblk, the last one unbounded
   int last_parts_ = 1;
+  // look-ahead Cholesky of a one-slot batch: the next panel's chain on the
+  // main stream beside the bulk trailing update on la_stream_ (persistent,
+  // la_workgroups_ = 2 per CU), ordered by the two events
+  hipStream_t la_stream_ = nullptr;
+  hipEvent_t la_ev_[2] = {nullptr, nullptr};
+  int la_workgroups_ = 512;
+  bool lookahead_ = false;
+  static constexpr int kLookaheadMinBlocks = 16;
   // allocation-free calls: the device scratch array and the host arrays
   // that stand in for the pinned parameter / result tables
   char* scratch_ = nullptr;
