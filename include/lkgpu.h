@@ -243,6 +243,16 @@ int lk_kriging_update(void* ptr, const double* y_u, int n, const double* noise_u
  * handle (NULL on error) whose factors are recomputed on the device */
 int lk_kriging_save(void* ptr, const char* filename);
 void* lk_kriging_load(const char* filename);
+/* libkriging_c.h:43 (Kriging(const Kriging&, ExplicitCopySpecifier),
+ * Kriging.cpp:164): a new independent handle with the same state (NULL on
+ * error) */
+void* lk_kriging_copy(void* ptr);
+/* libkriging_c.h:107 (Kriging::summary, Kriging.cpp:2660-2670) */
+const char* lk_kriging_summary(void* ptr);
+/* libkriging_c.h:138 (KrigingImpl::covMat, KrigingImpl.cpp:60-71):
+ * cov_out (n1 x n2, column-major) = sigma2 k(x1_i - x2_j) of the normalised
+ * inputs, computed on the device */
+int lk_kriging_cov_mat(void* ptr, const double* X1, int n1, int d1, const double* X2, int n2, int d2, double* cov_out);
 /* libkriging_c.h:133 (Kriging::leaveOneOut at the fitted theta) */
 double lk_kriging_leave_one_out(void* ptr);
 /* libkriging_c.h:136 (Kriging::leaveOneOutVec, Kriging.cpp:478-485): n
@@ -266,6 +276,20 @@ int lk_kriging_get_beta(void* ptr, double* out, int* n);
 int lk_kriging_get_theta(void* ptr, double* out, int* n);
 double lk_kriging_get_sigma2(void* ptr);
 double lk_kriging_get_nugget(void* ptr);
+/* libkriging_c.h:152-165: F (n x p trend matrix), T (n x n lower Cholesky
+ * factor of R), M = T^-1 F (n x p) and z (n) of the committed model -- T, M,
+ * z from one device evaluation at the committed parameters -- the
+ * estimation flags, and the heterogeneous noise vector (always empty here:
+ * the noise models of this path are none and nugget) */
+int lk_kriging_get_F(void* ptr, double* out, int* n, int* d);
+int lk_kriging_get_T(void* ptr, double* out, int* n, int* d);
+int lk_kriging_get_M(void* ptr, double* out, int* n, int* d);
+int lk_kriging_get_z(void* ptr, double* out, int* n);
+int lk_kriging_is_beta_estim(void* ptr);
+int lk_kriging_is_theta_estim(void* ptr);
+int lk_kriging_is_sigma2_estim(void* ptr);
+int lk_kriging_is_nugget_estim(void* ptr);
+int lk_kriging_get_noise(void* ptr, double* out, int* n);
 
 /* evaluations and batched launches of the last fit; no reference counterpart */
 int lk_kriging_fit_statistics(void* ptr, int* evaluations, int* batches);

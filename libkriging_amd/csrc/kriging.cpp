@@ -7,6 +7,7 @@
 #include <limits>
 #include <map>
 #include <mutex>
+#include <sstream>
 #include <stdexcept>
 
 #include "lbfgsb.hpp"
@@ -1079,6 +1080,92 @@ void Kriging::commit_apply(const std::vector<double>& theta, double alpha, int s
     if (est_theta_ && objective_ == "LMP") sigma2_ = sse / (n - p);
   }
   is_empty_ = false;
+}
+
+}  // namespace lk
+
+namespace lk {
+
+// KrigingImpl::summary_top / summary_bottom (KrigingImpl.cpp:713-762) and
+// Kriging::summary (Kriging.cpp:2660-2670)
+std::string Kriging::summary() const {
+  std::ostringstream oss;
+  auto vec = [&oss](const std::vector<double>& v) {
+    for (size_t i = 0; i < v.size(); ++i) oss << (i ? ", " : "") << v[i];
+  };
+  if (is_empty_ || n_ == 0) {
+    oss << "* covariance:\n";
+    oss << "  * kernel: " << kernel_ << "\n";
+    return oss.str();
+  }
+  oss << "* data" << (normalize_ ? " (normalized): " : ": ") << n_ << "x";
+  for (int k = 0; k < d_; ++k) {
+    // the displayed design is the original one
+    double mn = X_[(size_t)k * n_], mx = mn;
+    for (int i = 1; i < n_; ++i) {
+      mn = std::min(mn, X_[i + (size_t)k * n_]);
+      mx = std::max(mx, X_[i + (size_t)k * n_]);
+    }
+    oss << "[" << mn * scaleX_[k] + centerX_[k] << "," << mx * scaleX_[k] + centerX_[k] << "]" << (k + 1 < d_ ? "," : "");
+  }
+  const double ymn = *std::min_element(y_.begin(), y_.end()), ymx = *std::max_element(y_.begin(), y_.end());
+  oss << " -> " << n_ << "x[" << ymn * scaleY_ + centerY_ << "," << ymx * scaleY_ + centerY_ << "]\n";
+  oss << "* trend " << regmodel_name(regmodel_) << (est_beta_ ? " (est.): " : ": ");
+  vec(beta_);
+  oss << "\n* variance" << (est_sigma2_ ? " (est.): " : ": ") << sigma2_ << "\n";
+  oss << "* covariance:\n";
+  oss << "  * kernel: " << kernel_ << "\n";
+  oss << "  * range" << (est_theta_ ? " (est.): " : ": ");
+  vec(theta_);
+  oss << "\n";
+  if (noise_ == NoiseModel::Nugget) oss << "  * nugget" << (est_nugget_ ? " (est.): " : ": ") << nugget_ << "\n";
+  oss << "  * fit:\n";
+  oss << "    * objective: " << objective_ << "\n";
+  oss << "    * optim: " << optim_ << "\n";
+  return oss.str();
+}
+
+void Kriging::cov_mat(const double* X1, int n1, const double* X2, int n2, double* out) const {
+  if (is_empty_) throw std::runtime_error("model is not fitted");
+  if (n1 <= 0 || n2 <= 0) return;
+  const int d = d_;
+  // normalised inputs (KrigingImpl.cpp:61-66)
+  std::vector<double> a((size_t)n1 * d), b((size_t)n2 * d);
+  for (int k = 0; k < d; ++k) {
+    for (int i = 0; i < n1; ++i) a[i + (size_t)k * n1] = (X1[i + (size_t)k * n1] - centerX_[k]) / scaleX_[k];
+    for (int j = 0; j < n2; ++j) b[j + (size_t)k * n2] = (X2[j + (size_t)k * n2] - centerX_[k]) / scaleX_[k];
+  }
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  double *dA = nullptr, *dB = nullptr, *dO = nullptr;
+  auto cleanup = [&] {
+    if (dA) hipFree(dA);
+    if (dB) hipFree(dB);
+    if (dO) hipFree(dO);
+  };
+  try {
+    hip_check(hipMalloc(&dA, sizeof(double) * a.size()), "hipMalloc");
+    hip_check(hipMalloc(&dB, sizeof(double) * b.size()), "hipMalloc");
+    hip_check(hipMalloc(&dO, sizeof(double) * (size_t)n1 * n2), "hipMalloc");
+    hip_check(hipMemcpy(dA, a.data(), sizeof(double) * a.size(), hipMemcpyHostToDevice), "upload");
+    hip_check(hipMemcpy(dB, b.data(), sizeof(double) * b.size(), hipMemcpyHostToDevice), "upload");
+    CovMatArgs c{};
+    c.X1 = dA;
+    c.X2 = dB;
+    c.out = dO;
+    for (int k = 0; k < d; ++k) c.theta[k] = theta_[k];
+    c.scale = sigma2_;
+    c.n1 = n1;
+    c.n2 = n2;
+    c.d = d;
+    c.kernel = kernel_id(kernel_);
+    hip_check(launch_cov_mat(c, rt_->stream()), "cov_mat");
+    hip_check(hipStreamSynchronize(rt_->stream()), "sync");
+    hip_check(hipMemcpy(out, dO, sizeof(double) * (size_t)n1 * n2, hipMemcpyDeviceToHost), "download");
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
 }
 
 }  // namespace lk

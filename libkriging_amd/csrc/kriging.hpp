@@ -21,6 +21,8 @@ enum class NoiseModel { None = 0, Nugget = 1 };
 enum class RegModel { None = 0, Constant = 1, Linear = 2 };
 
 RegModel regmodel_from_string(const std::string& s);
+// Trend::toString (Trend.cpp:29-32): "none", "constant", "linear"
+const char* regmodel_name(RegModel r);
 int kernel_id(const std::string& name);
 
 // KrigingParameters (Kriging.hpp:18-28); theta is rows x d, row-major
@@ -105,6 +107,18 @@ class Kriging {
   // the device at the saved parameters
   void save(const std::string& filename);
   static Kriging* load(const std::string& filename);
+  std::string to_json() const;
+  static Kriging* from_json(const std::string& text, const std::string& filename);
+  // an independent model with this one's state (Kriging.cpp:164)
+  Kriging* copy() const;
+  // T = L, M = F*, z and R^-1 at the committed parameters (any may be null)
+  void factors(std::vector<double>* T, std::vector<double>* M, std::vector<double>* z,
+               std::vector<double>* Rinv) const;
+  // Kriging::summary (Kriging.cpp:2660-2670, KrigingImpl.cpp:713-762)
+  std::string summary() const;
+  // KrigingImpl::covMat (KrigingImpl.cpp:60-71): sigma2 k(x1_i - x2_j) of
+  // the normalised inputs, n1 x n2 (X1 n1 x d, X2 n2 x d column-major)
+  void cov_mat(const double* X1, int n1, const double* X2, int n2, double* out) const;
   // log marginal posterior (Kriging::logMargPostFun / logMargPost,
   // Kriging.cpp:488-672); gamma as logLikelihoodFun
   double logMargPostFun(const double* gamma, int gamma_n, bool want_grad, double* grad);
@@ -139,6 +153,11 @@ class Kriging {
   RegModel regmodel() const { return regmodel_; }
   const std::vector<double>& X() const { return X_; }
   const std::vector<double>& y() const { return y_; }
+  const std::vector<double>& F() const { return F_; }
+  bool is_beta_estim() const { return est_beta_; }
+  bool is_theta_estim() const { return est_theta_; }
+  bool is_sigma2_estim() const { return est_sigma2_; }
+  bool is_nugget_estim() const { return est_nugget_; }
   // statistics of the last fit
   int fit_evaluations() const { return fit_evals_; }
   int fit_batches() const { return fit_batches_; }

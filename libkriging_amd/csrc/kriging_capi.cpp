@@ -279,6 +279,82 @@ int lk_kriging_get_theta(void* ptr, double* out, int* n) { return copy_vec(K(ptr
 double lk_kriging_get_sigma2(void* ptr) { return K(ptr)->sigma2(); }
 double lk_kriging_get_nugget(void* ptr) { return K(ptr)->nugget(); }
 
+void* lk_kriging_copy(void* ptr) {
+  lk::Kriging* c = nullptr;
+  guarded([&] { c = K(ptr)->copy(); });
+  return c;
+}
+
+const char* lk_kriging_summary(void* ptr) {
+  std::string s;
+  if (guarded([&] { s = K(ptr)->summary(); }) != 0) return nullptr;
+  return ret_str(s);
+}
+
+int lk_kriging_cov_mat(void* ptr, const double* X1, int n1, int d1, const double* X2, int n2, int d2, double* cov_out) {
+  return guarded([&] {
+    lk::Kriging* k = K(ptr);
+    if (d1 != k->d() || d2 != k->d()) throw std::invalid_argument("X1 and X2 should have d columns");
+    if (n1 < 0 || n2 < 0 || (n1 > 0 && X1 == nullptr) || (n2 > 0 && X2 == nullptr) || cov_out == nullptr)
+      throw std::invalid_argument("NULL argument");
+    k->cov_mat(X1, n1, X2, n2, cov_out);
+  });
+}
+
+static int copy_mat(const std::vector<double>& v, int rows, int cols, double* out, int* n, int* d) {
+  if (n) *n = rows;
+  if (d) *d = cols;
+  if (out) std::memcpy(out, v.data(), sizeof(double) * (size_t)rows * cols);
+  return 0;
+}
+
+int lk_kriging_get_F(void* ptr, double* out, int* n, int* d) {
+  return guarded([&] { copy_mat(K(ptr)->F(), K(ptr)->n(), K(ptr)->p(), out, n, d); });
+}
+int lk_kriging_get_T(void* ptr, double* out, int* n, int* d) {
+  return guarded([&] {
+    lk::Kriging* k = K(ptr);
+    if (n) *n = k->n();
+    if (d) *d = k->n();
+    if (out == nullptr) return;
+    std::vector<double> T;
+    k->factors(&T, nullptr, nullptr, nullptr);
+    copy_mat(T, k->n(), k->n(), out, n, d);
+  });
+}
+int lk_kriging_get_M(void* ptr, double* out, int* n, int* d) {
+  return guarded([&] {
+    lk::Kriging* k = K(ptr);
+    if (n) *n = k->n();
+    if (d) *d = k->p();
+    if (out == nullptr) return;
+    std::vector<double> M;
+    k->factors(nullptr, &M, nullptr, nullptr);
+    copy_mat(M, k->n(), k->p(), out, n, d);
+  });
+}
+int lk_kriging_get_z(void* ptr, double* out, int* n) {
+  return guarded([&] {
+    lk::Kriging* k = K(ptr);
+    if (n) *n = k->n();
+    if (out == nullptr) return;
+    std::vector<double> z;
+    k->factors(nullptr, nullptr, &z, nullptr);
+    copy_vec(z, out, n);
+  });
+}
+int lk_kriging_is_beta_estim(void* ptr) { return K(ptr)->is_beta_estim() ? 1 : 0; }
+int lk_kriging_is_theta_estim(void* ptr) { return K(ptr)->is_theta_estim() ? 1 : 0; }
+int lk_kriging_is_sigma2_estim(void* ptr) { return K(ptr)->is_sigma2_estim() ? 1 : 0; }
+int lk_kriging_is_nugget_estim(void* ptr) { return K(ptr)->is_nugget_estim() ? 1 : 0; }
+int lk_kriging_get_noise(void* ptr, double* out, int* n) {
+  (void)out;
+  return guarded([&] {
+    K(ptr);
+    if (n) *n = 0;
+  });
+}
+
 int lk_kriging_fit_statistics(void* ptr, int* evaluations, int* batches) {
   return guarded([&] {
     lk::Kriging* k = K(ptr);

@@ -16,8 +16,6 @@
 
 namespace lk {
 
-namespace {
-
 const char* regmodel_name(RegModel r) {
   switch (r) {
     case RegModel::None:
@@ -28,6 +26,8 @@ const char* regmodel_name(RegModel r) {
       return "linear";
   }
 }
+
+namespace {
 
 // correlation of the covariance family (Covariance.cpp:24-161) at dx
 double corr(const std::string& kernel, const double* dx, const std::vector<double>& theta) {
@@ -53,30 +53,59 @@ double corr(const std::string& kernel, const double* dx, const std::vector<doubl
 
 }  // namespace
 
-void Kriging::save(const std::string& filename) {
+// T = L (n x n), M = F* (n x p), z = E* (or y* - M beta when beta is fixed,
+// Kriging.cpp:2170-2172) and, when asked, R^-1, from one device evaluation
+// at the committed parameters
+void Kriging::factors(std::vector<double>* T, std::vector<double>* M, std::vector<double>* z,
+                      std::vector<double>* Rinv) const {
   if (is_empty_) throw std::runtime_error("model is not fitted");
-  const int n = n_, d = d_, p = p_;
-  const bool nug = noise_ == NoiseModel::Nugget;
-  // factors at the committed parameters: L, F*, E* / y*, then R^-1
+  const int n = n_, p = p_;
   std::vector<double> g(theta_);
-  if (nug) g.push_back(alpha_);
+  if (noise_ == NoiseModel::Nugget) g.push_back(alpha_);
   std::vector<EvalRequest> reqs{commit_request(g)};
   reqs[0].flags.alpha = alpha_;
   std::vector<EvalResult> res;
   rt_->evaluate(reqs, false, res);
   if (res[0].status != EVAL_OK)
     throw std::runtime_error(chol_failure_message(res[0].status, res[0].inc, rt_->chol.num_nugget));
-  std::vector<double> T((size_t)n * n), Rinv((size_t)n * n), M((size_t)n * std::max(p, 1)), z(n);
-  rt_->read_slot(0, 0, T.data());
-  if (p > 0) rt_->read_slot(0, 2, M.data());
-  if (est_beta_) {
-    rt_->read_slot(0, 4, z.data());
-  } else {
-    rt_->read_slot(0, 3, z.data());  // z = y* - M beta (Kriging.cpp:2170-2172)
-    for (int i = 0; i < n; ++i)
-      for (int a = 0; a < p; ++a) z[i] -= M[i + (size_t)a * n] * beta_[a];
+  std::vector<double> Mv((size_t)n * std::max(p, 1));
+  if (T) {
+    T->assign((size_t)n * n, 0.0);
+    rt_->read_slot(0, 0, T->data());
   }
-  rt_->read_rinv(0, Rinv.data());
+  if (p > 0) rt_->read_slot(0, 2, Mv.data());
+  if (z) {
+    z->assign(n, 0.0);
+    if (est_beta_) {
+      rt_->read_slot(0, 4, z->data());
+    } else {
+      rt_->read_slot(0, 3, z->data());
+      for (int i = 0; i < n; ++i)
+        for (int a = 0; a < p; ++a) (*z)[i] -= Mv[i + (size_t)a * n] * beta_[a];
+    }
+  }
+  if (M) M->assign(Mv.begin(), Mv.begin() + (size_t)n * p);
+  if (Rinv) {
+    Rinv->assign((size_t)n * n, 0.0);
+    rt_->read_rinv(0, Rinv->data());
+  }
+}
+
+void Kriging::save(const std::string& filename) {
+  const std::string text = to_json();
+  std::ofstream f(filename, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open '" + filename + "' for writing");
+  f << text;
+  if (!f) throw std::runtime_error("cannot write '" + filename + "'");
+}
+
+std::string Kriging::to_json() const {
+  if (is_empty_) throw std::runtime_error("model is not fitted");
+  const int n = n_, d = d_, p = p_;
+  const bool nug = noise_ == NoiseModel::Nugget;
+  // factors at the committed parameters: L, F*, E* / y*, then R^-1
+  std::vector<double> T, M, z, Rinv;
+  factors(&T, &M, &z, &Rinv);
   // R (un-jittered: factor * k off the diagonal, 1 on it), dX and maxdX
   // (LinearAlgebra::compute_dX: column i n + j and j n + i hold x_i - x_j)
   const double factor = nug ? alpha_ : 1.0;
@@ -138,10 +167,7 @@ void Kriging::save(const std::string& filename) {
   }
   field("nystrom_light", "false", true);
   o << "}\n";
-  std::ofstream f(filename, std::ios::binary);
-  if (!f) throw std::runtime_error("cannot open '" + filename + "' for writing");
-  f << o.str();
-  if (!f) throw std::runtime_error("cannot write '" + filename + "'");
+  return o.str();
 }
 
 Kriging* Kriging::load(const std::string& filename) {
@@ -149,7 +175,15 @@ Kriging* Kriging::load(const std::string& filename) {
   if (!f) throw std::runtime_error("cannot open '" + filename + "'");
   std::stringstream buf;
   buf << f.rdbuf();
-  const io::Json j = io::parse(buf.str());
+  return from_json(buf.str(), filename);
+}
+
+// Kriging(const Kriging&, ExplicitCopySpecifier) (Kriging.cpp:164): the same
+// state through the model file's fields, the factors recomputed on the device
+Kriging* Kriging::copy() const { return from_json(to_json(), "(copy)"); }
+
+Kriging* Kriging::from_json(const std::string& text, const std::string& filename) {
+  const io::Json j = io::parse(text);
   const int version = (int)j.at("version").as_number();
   if (version != 2)
     throw std::runtime_error("Bad version to load from '" + filename + "'; found " + std::to_string(version) +

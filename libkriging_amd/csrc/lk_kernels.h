@@ -176,6 +176,16 @@ struct PredictArgs {
   int want_var;        // derivatives of the variance too
 };
 hipError_t launch_predict(const PredictArgs& a, hipStream_t stream);
+// covariance between two point sets: out(i, j) = scale k(x1_i - x2_j)
+struct CovMatArgs {
+  const double* X1;  // n1 x d, ld = n1
+  const double* X2;  // n2 x d, ld = n2
+  double* out;       // n1 x n2, ld = n1
+  double theta[DMAX];
+  double scale;
+  int n1, n2, d, kernel;
+};
+hipError_t launch_cov_mat(const CovMatArgs& a, hipStream_t stream);
 hipError_t launch_predict_cov(const PredictArgs& a, hipStream_t stream);    // after launch_predict (same chunk)
 hipError_t launch_predict_deriv(const PredictArgs& a, hipStream_t stream);  // after launch_predict (same chunk)
 // log-marginal-posterior gradient: the fused R^-1 / adjoint pass with

@@ -1153,6 +1153,50 @@ __global__ __launch_bounds__(TPB) void k_cross_cov(PredictArgs a) {
   }
 }
 
+// Covariance between two point sets, LinearAlgebra::covMat_rect
+// (LinearAlgebra.cpp:600-660) as used by KrigingImpl::covMat:
+// out(i, j) = scale k(x1_i - x2_j; theta), n1 x n2 column-major.
+// grid (ceil(n2 / 64), ceil(n1 / 64)), 256 threads
+template <int DT>
+__global__ __launch_bounds__(TPB) void k_cov_mat(CovMatArgs a) {
+  const int j0 = blockIdx.x * NB, i0 = blockIdx.y * NB, tid = threadIdx.x;
+  const int n1 = a.n1, n2 = a.n2, d = a.d;
+  __shared__ double s1[DMAX * NB];
+  __shared__ double s2[DMAX * NB];
+  for (int q = tid; q < d * NB; q += TPB) {
+    const int k = q / NB, r = q % NB;
+    s1[q] = (i0 + r < n1) ? a.X1[(size_t)k * n1 + i0 + r] : 0.0;
+    s2[q] = (j0 + r < n2) ? a.X2[(size_t)k * n2 + j0 + r] : 0.0;
+  }
+  double it[DT];
+#pragma unroll
+  for (int k = 0; k < DT; ++k) it[k] = (k < d) ? 1.0 / a.theta[k] : 1.0;
+  __syncthreads();
+  const int ii = tid & 63, gi = i0 + ii;
+  double dx[DT];
+  for (int r = 0; r < NB / 4; ++r) {
+    const int jj = (tid >> 6) + 4 * r, gj = j0 + jj;
+    if (gi >= n1 || gj >= n2) continue;
+#pragma unroll
+    for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? s1[k * NB + ii] - s2[k * NB + jj] : 0.0;
+    a.out[(size_t)gj * n1 + gi] = kern_rho<DT>(a.kernel, dx, it) * a.scale;
+  }
+}
+
+hipError_t launch_cov_mat(const CovMatArgs& a, hipStream_t stream) {
+  if (a.n1 <= 0 || a.n2 <= 0) return hipSuccess;
+  const dim3 grid((a.n2 + NB - 1) / NB, (a.n1 + NB - 1) / NB);
+  if (a.d <= 4)
+    hipLaunchKernelGGL(k_cov_mat<4>, grid, dim3(TPB), 0, stream, a);
+  else if (a.d <= 8)
+    hipLaunchKernelGGL(k_cov_mat<8>, grid, dim3(TPB), 0, stream, a);
+  else if (a.d <= 16)
+    hipLaunchKernelGGL(k_cov_mat<16>, grid, dim3(TPB), 0, stream, a);
+  else
+    hipLaunchKernelGGL(k_cov_mat<32>, grid, dim3(TPB), 0, stream, a);
+  return hipGetLastError();
+}
+
 // C(it, jt) = sum_{kt <= it} Linv(it, kt) R_on(kt, jt); grid (mpad / 64, npad / 64)
 __global__ __launch_bounds__(TPB) void k_trmm_lower(PredictArgs a) {
   const int jt = blockIdx.x, it = blockIdx.y;

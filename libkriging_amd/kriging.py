@@ -54,6 +54,17 @@ for _name in ("lk_kriging_get_centerY", "lk_kriging_get_scaleY", "lk_kriging_get
     N.register(_name, ctypes.c_double, [_vp])
 N.register("lk_kriging_fit_statistics", ctypes.c_int, [_vp, _ip, _ip])
 N.register("lk_kriging_set_batched_fit", ctypes.c_int, [_vp, ctypes.c_int])
+N.register("lk_kriging_copy", _vp, [_vp])
+N.register("lk_kriging_summary", ctypes.c_char_p, [_vp])
+N.register("lk_kriging_cov_mat", ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int,
+                                                _dp])
+for _name in ("lk_kriging_get_F", "lk_kriging_get_T", "lk_kriging_get_M"):
+    N.register(_name, ctypes.c_int, [_vp, _dp, _ip, _ip])
+for _name in ("lk_kriging_get_z", "lk_kriging_get_noise"):
+    N.register(_name, ctypes.c_int, [_vp, _dp, _ip])
+for _name in ("lk_kriging_is_beta_estim", "lk_kriging_is_theta_estim", "lk_kriging_is_sigma2_estim",
+              "lk_kriging_is_nugget_estim"):
+    N.register(_name, ctypes.c_int, [_vp])
 N.register("lk_optim_set", ctypes.c_int, [_cp, ctypes.c_double])
 N.register("lk_optim_get", ctypes.c_int, [_cp, _dp])
 N.register("lk_linalg_set", ctypes.c_int, [_cp, ctypes.c_double])
@@ -344,6 +355,73 @@ class Kriging:
         e, b = ctypes.c_int(), ctypes.c_int()
         N.check(N.lib().lk_kriging_fit_statistics(self._h, ctypes.byref(e), ctypes.byref(b)))
         return e.value, b.value
+
+    def copy(self):
+        """an independent model with this one's state (pylibkriging copy)"""
+        h = N.lib().lk_kriging_copy(self._h)
+        if not h:
+            raise RuntimeError(N.lib().lk_get_last_error().decode())
+        k = Kriging.__new__(Kriging)
+        k._h = h
+        k._keep = None
+        return k
+
+    def summary(self):
+        s = N.lib().lk_kriging_summary(self._h)
+        if s is None:
+            raise RuntimeError(N.lib().lk_get_last_error().decode())
+        return s.decode()
+
+    def __repr__(self):
+        return self.summary()
+
+    def covMat(self, X1, X2):
+        """sigma2 k(x1_i - x2_j) (KrigingImpl::covMat), n1 x n2"""
+        X1, X2 = _mat(X1), _mat(X2)
+        out = np.zeros((X1.shape[0], X2.shape[0]), order="F")
+        N.check(N.lib().lk_kriging_cov_mat(self._h, N.ptr(X1), X1.shape[0], X1.shape[1], N.ptr(X2), X2.shape[0],
+                                           X2.shape[1], N.ptr(out)))
+        return out
+
+    def _mat_get(self, fn):
+        n, d = ctypes.c_int(), ctypes.c_int()
+        N.check(getattr(N.lib(), fn)(self._h, None, ctypes.byref(n), ctypes.byref(d)))
+        out = np.zeros((n.value, d.value), order="F")
+        N.check(getattr(N.lib(), fn)(self._h, N.ptr(out), ctypes.byref(n), ctypes.byref(d)))
+        return out
+
+    def F(self):
+        return self._mat_get("lk_kriging_get_F")
+
+    def T(self):
+        return self._mat_get("lk_kriging_get_T")
+
+    def M(self):
+        return self._mat_get("lk_kriging_get_M")
+
+    def z(self):
+        n = ctypes.c_int()
+        N.check(N.lib().lk_kriging_get_z(self._h, None, ctypes.byref(n)))
+        out = np.zeros(n.value)
+        N.check(N.lib().lk_kriging_get_z(self._h, N.ptr(out), ctypes.byref(n)))
+        return out
+
+    def noise(self):
+        n = ctypes.c_int()
+        N.check(N.lib().lk_kriging_get_noise(self._h, None, ctypes.byref(n)))
+        return np.zeros(n.value)
+
+    def is_beta_estim(self):
+        return bool(N.lib().lk_kriging_is_beta_estim(self._h))
+
+    def is_theta_estim(self):
+        return bool(N.lib().lk_kriging_is_theta_estim(self._h))
+
+    def is_sigma2_estim(self):
+        return bool(N.lib().lk_kriging_is_sigma2_estim(self._h))
+
+    def is_nugget_estim(self):
+        return bool(N.lib().lk_kriging_is_nugget_estim(self._h))
 
     def set_batched_fit(self, batched):
         N.check(N.lib().lk_kriging_set_batched_fit(self._h, int(bool(batched))))

@@ -166,3 +166,51 @@ def test_update_rejects_a_wrong_dimension():
     k = lk.Kriging(y, X, "gauss", "constant", False, "none", "LL", {"theta": np.array([[0.5, 0.5]])})
     with pytest.raises(RuntimeError, match="Dimension of new data"):
         k.update(np.ones(3), np.ones((3, 3)), False)
+
+
+def test_copy_summary_and_getters():
+    """lk_kriging_copy (an independent model), summary, covMat
+    (KrigingImpl.cpp:60-71, normalised inputs, sigma2 k) and the factor
+    getters T, M, z (libkriging_c.h:152-165) against the oracle's
+    populate_Model at the committed parameters"""
+    import libkriging_amd as lk
+    from oracle import oracle as O
+    rng = np.random.default_rng(17)
+    X = rng.uniform(size=(90, 3)) * np.array([2.0, 1.0, 0.5]) + 1.0
+    y = np.sin(2 * X).sum(axis=1)
+    k = lk.Kriging(y, X, "matern5_2", "linear", True, "BFGS", "LL")
+    c = k.copy()
+    assert np.array_equal(c.theta(), k.theta()) and c.sigma2() == k.sigma2()
+    assert c.logLikelihood() == k.logLikelihood()
+    Xn = rng.uniform(size=(20, 3)) + 1.0
+    assert np.array_equal(c.predict(Xn)[0], k.predict(Xn)[0])
+    c.update(y[:5] + 0.1, X[:5] + 0.01, False)  # the copy changes, the original does not
+    assert k.X().shape[0] == 90 and c.X().shape[0] == 95
+    s = k.summary()
+    print(s)
+    assert "* data (normalized): 90x" in s and "matern5_2" in s and "* trend linear (est.)" in s
+    assert "range (est.)" in s and "objective: LL" in s
+    # covMat: sigma2 k(x1 - x2) of the normalised inputs
+    cm = k.covMat(X[:7], Xn)
+    Xs, Xns = (X[:7] - k.centerX()) / k.scaleX(), (Xn - k.centerX()) / k.scaleX()
+    ref = k.sigma2() * O.corr_mat("matern5_2", Xs, Xns, k.theta())
+    assert np.max(np.abs(cm - ref)) <= 1e-14 * k.sigma2()
+    # T, M, z, F and the flags
+    P = O.OracleProblem(k.X(), k.y(), "matern5_2", "linear")
+    m = P.populate_model(k.theta())
+    T, M, z = k.T(), k.M(), k.z()
+    assert T.shape == (90, 90) and M.shape == (90, 4) and z.shape == (90,)
+    assert np.linalg.norm(T - np.tril(m["L"])) <= 1e-12 * np.linalg.norm(m["L"])
+    assert np.linalg.norm(M - m["Fstar"]) <= 1e-12 * np.linalg.norm(m["Fstar"])
+    assert np.linalg.norm(z - m["Estar"]) <= 1e-10 * np.linalg.norm(m["Estar"])
+    assert np.array_equal(k.F(), O.trend_matrix(k.X(), "linear"))
+    assert k.is_beta_estim() and k.is_theta_estim() and k.is_sigma2_estim()
+    assert k.noise().size == 0
+    kf = lk.Kriging(y, X, "gauss", "constant", False, "none", "LL",
+                    {"theta": np.array([[0.5, 0.5, 0.5]]), "is_theta_estim": False, "sigma2": 2.0,
+                     "is_sigma2_estim": False, "beta": [0.3], "is_beta_estim": False})
+    assert not (kf.is_beta_estim() or kf.is_theta_estim() or kf.is_sigma2_estim())
+    # beta fixed: z = y* - M beta (Kriging.cpp:2170-2172)
+    mf = O.OracleProblem(X, y, "gauss").populate_model(np.array([0.5, 0.5, 0.5]))
+    zf = mf["ystar"] - mf["Fstar"][:, 0] * 0.3
+    assert np.linalg.norm(kf.z() - zf) <= 1e-10 * np.linalg.norm(zf)
