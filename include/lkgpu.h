@@ -309,6 +309,31 @@ int lk_kriging_set_batched_fit(void* ptr, int batched);
 int lk_optim_set(const char* name, double value);
 int lk_optim_get(const char* name, double* value);
 int lk_linalg_set(const char* name, double value);
+/* the current value of a LinearAlgebra setting (as lk_linalg_set names
+ * them): get_num_nugget, chol_rcond_checked (LinearAlgebra.cpp:37-62) */
+int lk_linalg_get(const char* name, double* value);
+
+/* The public LinearAlgebra routines the R binding exports
+ * (bindings/R/rlibkriging/src/linalg.cpp:39-71), on the default device.
+ * Matrices are n x n column-major host buffers.
+ * lk_linalg_chol_safe: LinearAlgebra::safe_chol_lower (LinearAlgebra.cpp:
+ *   43-98): L (lower, zero above) of X + the retry ladder's cumulative
+ *   jitter (checked against rcond when chol_rcond_check is on); inc_out (may
+ *   be NULL) receives the jitter steps; -1 with the reference's messages
+ *   when the ladder is exhausted or num_nugget is not positive.
+ * lk_linalg_rcond_chol: LinearAlgebra::rcond_chol (LinearAlgebra.cpp:
+ *   106-115), rcond(L)^2 by the 1-norm estimator of LAPACK's dtrcon, for
+ *   the lower triangle of L (0 for a zero diagonal entry).
+ * lk_linalg_rcond_approx_chol: LinearAlgebra::rcond_approx_chol
+ *   (LinearAlgebra.cpp:120-139), (min / max of L's diagonal)^2.
+ * lk_linalg_chol_block: LinearAlgebra::chol_block (LinearAlgebra.cpp:
+ *   257-299): the factor of C from Loo (no x no), the factor of C's leading
+ *   block (diagonal >= 0); L_ou = Loo^-1 C_ou, L_uu = safe_chol(C_uu -
+ *   L_ou' L_ou), and safe_chol(C) when that fails. */
+int lk_linalg_chol_safe(const double* X, int n, double* L_out, int* inc_out);
+int lk_linalg_rcond_chol(const double* L, int n, double* rcond_out);
+int lk_linalg_rcond_approx_chol(const double* L, int n, double* rcond_out);
+int lk_linalg_chol_block(const double* C, int n, const double* Loo, int no, double* L_out);
 int lk_random_reset_seed(unsigned seed);
 
 /* The bound-constrained optimiser the fit uses, on a user objective

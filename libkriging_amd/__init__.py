@@ -12,6 +12,7 @@ adjoint gradient).
 from ._native import Context, ModelFlags  # noqa: F401
 from .kriging import Kriging, Optim, lbfgsb_minimize, set_num_nugget, check_chol_rcond, reset_seed  # noqa: F401
 from .nested import NestedKriging  # noqa: F401
+from . import linalg  # noqa: F401
 
 __version__ = "0.1.0"
 

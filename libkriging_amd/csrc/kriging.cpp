@@ -109,6 +109,7 @@ Kriging::~Kriging() {
       } catch (...) {
       }
     }
+    if (d_L_) hipFree(d_L_);
     if (d_Linv_) hipFree(d_Linv_);
     if (d_vec_) hipFree(d_vec_);
   }
@@ -836,13 +837,55 @@ void Kriging::update(const double* y_u, const double* X_u, int m, bool refit) {
     return;
   }
   // no re-fit: the model at the current parameters on the extended design,
-  // sigma2 = SSE / n when estimated (KrigingImpl.cpp:576-632)
+  // sigma2 = SSE / n when estimated (KrigingImpl.cpp:576-632).  With new
+  // points, the factor is the block update of the committed one
+  // (Kriging.cpp:172-187 update_eligible -> LinearAlgebra::update_cholCov,
+  // LinearAlgebra.cpp:206-247): R on the extended design (factor * k off the
+  // diagonal, 1 on it), L = chol_block(R, T_old); the model follows from L
+  // in the factor mode of the pipeline
   std::vector<double> g(theta_);
   if (noise_ == NoiseModel::Nugget) g.push_back(alpha_);
   std::vector<EvalRequest> reqs{commit_request(g)};
   reqs[0].flags.alpha = alpha_;
   std::vector<EvalResult> res;
-  rt_->evaluate(reqs, false, res);
+  double *dR = nullptr, *dLn = nullptr;
+  struct Free {
+    double** p[2];
+    ~Free() {
+      for (double** q : p)
+        if (*q) hipFree(*q);
+    }
+  } fr{{&dR, &dLn}};
+  if (m > 0 && d_L_ != nullptr && d_Linv_ != nullptr) {
+    hip_check(hipSetDevice(device_), "hipSetDevice");
+    const size_t bn = sizeof(double) * (size_t)N * N;
+    hip_check(hipMalloc(&dR, bn), "hipMalloc");
+    hip_check(hipMalloc(&dLn, bn), "hipMalloc");
+    const DeviceProblem& P = rt_->problem(problem_);
+    CovMatArgs c{};
+    c.X1 = P.X;
+    c.X2 = P.X;
+    c.out = dR;
+    for (int k = 0; k < d; ++k) c.theta[k] = theta_[k];
+    c.scale = (noise_ == NoiseModel::Nugget) ? alpha_ : 1.0;
+    c.n1 = c.n2 = N;
+    c.d = d;
+    c.kernel = kernel_id(kernel_);
+    c.unit_diag = 1;
+    hip_check(launch_cov_mat(c, rt_->stream()), "R");
+    rt_->chol_block_device(dR, N, d_L_, n, d_Linv_, npad_, n, dLn);
+    rt_->set_problem_factor(problem_, dLn);
+    reqs[0].mode = MODE_FACTOR;
+    try {
+      rt_->evaluate(reqs, false, res);
+    } catch (...) {
+      rt_->set_problem_factor(problem_, nullptr);
+      throw;
+    }
+    rt_->set_problem_factor(problem_, nullptr);
+  } else {
+    rt_->evaluate(reqs, false, res);
+  }
   const double s2 = sigma2_, nug = nugget_;
   const bool est_theta = est_theta_;
   commit_apply(theta_, alpha_, 0, res[0]);
@@ -1014,10 +1057,16 @@ void Kriging::commit_apply(const std::vector<double>& theta, double alpha, int s
   const Runtime::SlotView v = rt_->slot_view(slot);
   npad_ = v.npad;
   hip_check(hipSetDevice(device_), "hipSetDevice");
+  if (d_L_) hipFree(d_L_);
   if (d_Linv_) hipFree(d_Linv_);
   if (d_vec_) hipFree(d_vec_);
+  d_L_ = nullptr;
   d_Linv_ = nullptr;
   d_vec_ = nullptr;
+  hip_check(hipMalloc(&d_L_, sizeof(double) * (size_t)n_ * n_), "hipMalloc model");
+  hip_check(hipMemcpy2DAsync(d_L_, sizeof(double) * n_, v.A, sizeof(double) * npad_, sizeof(double) * n_, n_,
+                             hipMemcpyDeviceToDevice, rt_->stream()),
+            "copy model");
   hip_check(hipMalloc(&d_Linv_, sizeof(double) * (size_t)npad_ * npad_), "hipMalloc model");
   hip_check(hipMalloc(&d_vec_, sizeof(double) * (size_t)npad_ * (p + 1)), "hipMalloc model");
   hip_check(hipMemcpyAsync(d_Linv_, v.W, sizeof(double) * (size_t)npad_ * npad_, hipMemcpyDeviceToDevice,

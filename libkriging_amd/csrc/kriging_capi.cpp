@@ -417,6 +417,61 @@ int lk_linalg_set(const char* name, double value) {
   });
 }
 
+int lk_linalg_get(const char* name, double* value) {
+  return guarded([&] {
+    if (value == nullptr) throw std::invalid_argument("NULL argument");
+    lk::Runtime& rt = lk::runtime_for(lk::default_device());
+    const std::string s = name ? name : "";
+    if (s == "num_nugget") *value = rt.chol.num_nugget;
+    else if (s == "chol_rcond_check") *value = rt.chol.rcond_check ? 1.0 : 0.0;
+    else if (s == "max_inc_choldiag") *value = rt.chol.max_inc;
+    else if (s == "min_rcond") *value = rt.chol.min_rcond;
+    else throw std::invalid_argument("unknown LinearAlgebra setting: " + s);
+  });
+}
+
+int lk_linalg_chol_safe(const double* X, int n, double* L_out, int* inc_out) {
+  return guarded([&] {
+    if (X == nullptr || L_out == nullptr) throw std::invalid_argument("NULL argument");
+    if (n <= 0) throw std::invalid_argument("matrix must have at least one row");
+    int inc = 0;
+    lk::runtime_for(lk::default_device()).safe_chol(X, n, L_out, &inc);
+    if (inc_out) *inc_out = inc;
+  });
+}
+
+int lk_linalg_rcond_chol(const double* L, int n, double* rcond_out) {
+  return guarded([&] {
+    if (L == nullptr || rcond_out == nullptr) throw std::invalid_argument("NULL argument");
+    if (n <= 0) throw std::invalid_argument("matrix must have at least one row");
+    *rcond_out = lk::runtime_for(lk::default_device()).rcond_chol(L, n);
+  });
+}
+
+int lk_linalg_rcond_approx_chol(const double* L, int n, double* rcond_out) {
+  return guarded([&] {
+    if (L == nullptr || rcond_out == nullptr) throw std::invalid_argument("NULL argument");
+    if (n <= 0) throw std::invalid_argument("matrix must have at least one row");
+    // (min / max of the diagonal)^2, the scan of LinearAlgebra.cpp:120-139
+    double lo = L[0], hi = L[0];
+    for (int i = 1; i < n; ++i) {
+      const double c = L[i + (size_t)i * n];
+      if (c < lo) lo = c;
+      else if (c > hi) hi = c;
+    }
+    const double r = lo / hi;
+    *rcond_out = r * r;
+  });
+}
+
+int lk_linalg_chol_block(const double* C, int n, const double* Loo, int no, double* L_out) {
+  return guarded([&] {
+    if (C == nullptr || Loo == nullptr || L_out == nullptr) throw std::invalid_argument("NULL argument");
+    if (n <= 0) throw std::invalid_argument("matrix must have at least one row");
+    lk::runtime_for(lk::default_device()).chol_block(C, n, Loo, no, L_out);
+  });
+}
+
 int lk_random_reset_seed(unsigned seed) {
   return guarded([&] { lk::random_reset_seed(seed); });
 }

@@ -55,7 +55,8 @@ double corr(const std::string& kernel, const double* dx, const std::vector<doubl
 
 // T = L (n x n), M = F* (n x p), z = E* (or y* - M beta when beta is fixed,
 // Kriging.cpp:2170-2172) and, when asked, R^-1, from one device evaluation
-// at the committed parameters
+// at the committed parameters that takes the committed factor L as given
+// (the reference keeps m_T, which a block update leaves as chol_block made it)
 void Kriging::factors(std::vector<double>* T, std::vector<double>* M, std::vector<double>* z,
                       std::vector<double>* Rinv) const {
   if (is_empty_) throw std::runtime_error("model is not fitted");
@@ -65,7 +66,17 @@ void Kriging::factors(std::vector<double>* T, std::vector<double>* M, std::vecto
   std::vector<EvalRequest> reqs{commit_request(g)};
   reqs[0].flags.alpha = alpha_;
   std::vector<EvalResult> res;
-  rt_->evaluate(reqs, false, res);
+  if (d_L_ != nullptr) {
+    rt_->set_problem_factor(problem_, d_L_);
+    reqs[0].mode = MODE_FACTOR;
+  }
+  try {
+    rt_->evaluate(reqs, false, res);
+  } catch (...) {
+    if (d_L_ != nullptr) rt_->set_problem_factor(problem_, nullptr);
+    throw;
+  }
+  if (d_L_ != nullptr) rt_->set_problem_factor(problem_, nullptr);
   if (res[0].status != EVAL_OK)
     throw std::runtime_error(chol_failure_message(res[0].status, res[0].inc, rt_->chol.num_nugget));
   std::vector<double> Mv((size_t)n * std::max(p, 1));

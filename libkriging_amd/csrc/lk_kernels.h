@@ -33,6 +33,7 @@ struct Slot {
   double* W;        // npad x npad: L^-1 in the lower triangle; scratch above
   double* V;        // npad x (p + 4): [F* | y* | r | E* | x]
   double* red;      // reduction scratch (see RED_* offsets)
+  const double* M;  // n x n (ld = n) input matrix of the matrix / factor modes, else null
   int n, npad, nblk, d, p;
   int kernel, noise;
   int pad_;
@@ -54,7 +55,17 @@ struct SlotParams {
   int want_grad;
   int gamma_has_alpha; // gradient also over alpha
   int objective;       // OBJ_LL, OBJ_LOO or OBJ_LMP (batch-wide)
+  int mode;            // MODE_KERNEL, MODE_MATRIX or MODE_FACTOR
+  int pad_;
 };
+
+// What a slot's matrix A starts as.  MODE_KERNEL: R built from the design
+// (the likelihood path).  MODE_MATRIX: the lower triangle of the slot's input
+// matrix M, jitter ladder on its diagonal (LinearAlgebra::safe_chol_lower).
+// MODE_FACTOR: M is already a lower Cholesky factor: no factorisation, the
+// inverse, solves and the condition estimate (always the 1-norm estimator,
+// as arma::rcond) from it.
+enum SlotMode { MODE_KERNEL = 0, MODE_MATRIX = 1, MODE_FACTOR = 2 };
 
 // Per-slot outputs (downloaded after each batch).
 struct SlotOut {
@@ -184,8 +195,34 @@ struct CovMatArgs {
   double theta[DMAX];
   double scale;
   int n1, n2, d, kernel;
+  int unit_diag;  // X1 == X2: out(i, i) = 1 (the correlation matrix's diagonal)
+  int pad_;
 };
 hipError_t launch_cov_mat(const CovMatArgs& a, hipStream_t stream);
+// C = Linv R_on alone (PredictArgs' Linv, Ron, C, npad, mpad; both padded
+// to whole 64-row blocks, Linv's diagonal blocks zero above the diagonal)
+hipError_t launch_trmm_lower(const PredictArgs& a, hipStream_t stream);
+// Schur complement of a block Cholesky update: S = Cuu - U' U, with U
+// kdim x mpad (ld = kdim, both padded to whole 64-row blocks, zero padding),
+// Cuu and S m x m (ld = m); S is written in full (both triangles)
+struct SchurArgs {
+  const double* U;
+  const double* Cuu;
+  double* S;
+  int kdim, m, mpad, pad_;
+};
+hipError_t launch_schur(const SchurArgs& a, hipStream_t stream);
+// L (n x n, ld = n) of a block Cholesky update: Loo's lower triangle (no x
+// no, ld = ldo), U' below it (U no x (n - no), ld = ldu), Luu's lower
+// triangle (ld = ldl) in the trailing block, zeros above the diagonal
+struct BlockAssembleArgs {
+  const double* Loo;
+  const double* U;
+  const double* Luu;
+  double* L;
+  int n, no, ldo, ldu, ldl, pad_;
+};
+hipError_t launch_block_assemble(const BlockAssembleArgs& a, hipStream_t stream);
 hipError_t launch_predict_cov(const PredictArgs& a, hipStream_t stream);    // after launch_predict (same chunk)
 hipError_t launch_predict_deriv(const PredictArgs& a, hipStream_t stream);  // after launch_predict (same chunk)
 // log-marginal-posterior gradient: the fused R^-1 / adjoint pass with

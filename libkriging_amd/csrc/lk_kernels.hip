@@ -51,6 +51,36 @@ __global__ __launch_bounds__(TPB) void k_build(const Slot* __restrict__ slots, c
   tri_decode(ts.tile, ti, tj);
   const int i0 = ti * NB, j0 = tj * NB;
   const int n = S.n, d = S.d, tid = threadIdx.x;
+  if (P.mode != MODE_KERNEL) {
+    // the input matrix's lower triangle (diagonal tiles: zero above), the
+    // padded part the identity; the matrix mode adds the retry ladder's
+    // cumulative jitter to the diagonal in the reference's order
+    // (LinearAlgebra.cpp:88: X.diag() += num_nugget * 10^inc, per retry)
+    const int ii = tid & 63, gi = i0 + ii;
+    for (int r = 0; r < NB / 4; ++r) {
+      const int jj = (tid >> 6) + 4 * r, gj = j0 + jj;
+      double v;
+      if (gi >= n || gj >= n) {
+        v = (gi == gj) ? 1.0 : 0.0;
+      } else if (gi < gj) {
+        v = 0.0;
+      } else {
+        v = S.M[(size_t)gj * n + gi];
+        // a given factor's columns are taken with a positive diagonal: the
+        // 1-norm condition number does not change with column signs
+        if (P.mode == MODE_FACTOR && S.M[(size_t)gj * n + gj] < 0.0) v = -v;
+        if (gi == gj && P.mode == MODE_MATRIX) {
+          double p10 = 1.0;
+          for (int q = 0; q < P.inc; ++q) {
+            v += P.num_nugget * p10;
+            p10 *= 10.0;
+          }
+        }
+      }
+      S.A[(size_t)gj * S.npad + gi] = v;
+    }
+    return;
+  }
 
   __shared__ double sXi[DMAX * NB];
   __shared__ double sXj[DMAX * NB];
@@ -300,11 +330,23 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(const Slot* __restrict__ slo
 #pragma unroll
   for (int q = 0; q < 16; ++q) a[q] = Ab[(size_t)(16 * w + q) * ld + l];
   bool ok;
-  switch (w) {
-    case 0: ok = diag_factor<0>(a, sh, l); break;
-    case 1: ok = diag_factor<1>(a, sh, l); break;
-    case 2: ok = diag_factor<2>(a, sh, l); break;
-    default: ok = diag_factor<3>(a, sh, l); break;
+  if (params[b].mode == MODE_FACTOR) {
+    // the block is already L_kk: its diagonal and reciprocals for the
+    // inverse sweep; a non-positive or NaN diagonal entry is a failure
+    if (w == (l >> 4)) {
+      sh.diag[l] = a[l & 15];
+      sh.rinv[l] = 1.0 / a[l & 15];
+    }
+    __syncthreads();
+    ok = true;
+    for (int q = 0; q < NB; ++q) ok = ok && (sh.diag[q] > 0.0);
+  } else {
+    switch (w) {
+      case 0: ok = diag_factor<0>(a, sh, l); break;
+      case 1: ok = diag_factor<1>(a, sh, l); break;
+      case 2: ok = diag_factor<2>(a, sh, l); break;
+      default: ok = diag_factor<3>(a, sh, l); break;
+    }
   }
   // every lane saw every pivot, so ok is uniform; no early exit before the
   // sweeps' barriers
@@ -376,7 +418,7 @@ __global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ sl
   const int b = ts.slot;
   const Slot S = slots[b];
   const int r = k + 1 + ts.tile;
-  if (!slot_live(params[b], out[b]) || r >= S.nblk) return;
+  if (!slot_live(params[b], out[b]) || r >= S.nblk || params[b].mode == MODE_FACTOR) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
   const int i0 = r * NB, k0 = k * NB, ld = S.npad;
   __shared__ double sbuf[4 * KC * LDSS];
@@ -448,7 +490,7 @@ __global__ __launch_bounds__(TPB) void k_potrf_update(const Slot* __restrict__ s
   const TileSlot ts = tile_slot();
   const int b = ts.slot;
   const Slot S = slots[b];
-  if (!slot_live(params[b], out[b])) return;
+  if (!slot_live(params[b], out[b]) || params[b].mode == MODE_FACTOR) return;
   const int nb = S.nblk;
   if (c0 >= nb || ke > nb) return;
   int r, c;
@@ -468,7 +510,7 @@ __global__ __launch_bounds__(TPB) void k_potrf_update_persistent(const Slot* __r
                                                                  SlotOut* __restrict__ out, int kb, int ke, int c0,
                                                                  int c1, int tiles) {
   const Slot S = slots[0];
-  if (!slot_live(params[0], out[0])) return;
+  if (!slot_live(params[0], out[0]) || params[0].mode == MODE_FACTOR) return;
   const int nb = S.nblk;
   if (c0 >= nb || ke > nb) return;
   __shared__ double sbuf[4 * KC * LDSS];
@@ -719,6 +761,8 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
   O.rcond2 = rc * rc;
   if (P.rcond_check && !(O.rcond2 >= P.min_rcond) && O.status == ST_OK)
     O.status = ST_RCOND_SUSPECT;
+  // a given factor: rcond from the 1-norm estimator, as arma::rcond
+  if (P.mode == MODE_FACTOR && O.status == ST_OK) O.status = ST_RCOND_SUSPECT;
   double ll, s2g;
   if (P.objective == OBJ_LMP) {
     // log marginal likelihood of Kriging::_logMargPost (Kriging.cpp:560-583),
@@ -1105,7 +1149,7 @@ __global__ __launch_bounds__(TPB) void k_rcond_estimate(const Slot* __restrict__
   if (tid == 0) {
     const double rc = (1.0 / O.norm1_L) / est;
     O.rcond2 = rc * rc;
-    O.status = (O.rcond2 < P.min_rcond) ? ST_RCOND_LOW : ST_OK;
+    O.status = (P.mode != MODE_FACTOR && O.rcond2 < P.min_rcond) ? ST_RCOND_LOW : ST_OK;
   }
 }
 
@@ -1179,7 +1223,7 @@ __global__ __launch_bounds__(TPB) void k_cov_mat(CovMatArgs a) {
     if (gi >= n1 || gj >= n2) continue;
 #pragma unroll
     for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? s1[k * NB + ii] - s2[k * NB + jj] : 0.0;
-    a.out[(size_t)gj * n1 + gi] = kern_rho<DT>(a.kernel, dx, it) * a.scale;
+    a.out[(size_t)gj * n1 + gi] = (a.unit_diag && gi == gj) ? 1.0 : kern_rho<DT>(a.kernel, dx, it) * a.scale;
   }
 }
 
@@ -1208,6 +1252,71 @@ __global__ __launch_bounds__(TPB) void k_trmm_lower(PredictArgs a) {
   // Linv(i0 + ii, k) R_on(k, j0 + jj) over k < (it + 1) * NB
   tile_mainloop<false, true, false>(a.Linv, ld, i0, a.Ron, ld, j0, 0, (it + 1) * NB, sbuf, acc, tid, lane, wr, wc);
   acc_store(acc, a.C + (size_t)j0 * ld + i0, ld, lane, wr, wc);
+}
+
+hipError_t launch_trmm_lower(const PredictArgs& a, hipStream_t stream) {
+  if (a.mpad <= 0 || a.npad <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trmm_lower, dim3(a.mpad / NB, a.npad / NB), dim3(TPB), 0, stream, a);
+  return hipGetLastError();
+}
+
+// S = Cuu - U' U over the lower tiles of S (grid T (T + 1) / 2, T = mpad /
+// 64); an off-diagonal tile also writes its transpose
+__global__ __launch_bounds__(TPB) void k_schur(SchurArgs a) {
+  int ti, tj;
+  tri_decode(blockIdx.x, ti, tj);
+  const int i0 = ti * NB, j0 = tj * NB;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  __shared__ double sbuf[4 * KC * LDSS];
+  d4 acc[2][2];
+  acc_zero(acc);
+  tile_mainloop<true, true, true>(a.U, a.kdim, i0, a.U, a.kdim, j0, 0, a.kdim, sbuf, acc, tid, lane, wr, wc);
+  const int m = a.m;
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int gi = i0 + acc_row(lane, wr, bi), gj = j0 + acc_col(lane, wc, bj, e);
+        if (gi >= m || gj >= m || (ti == tj && gi < gj)) continue;
+        const double v = a.Cuu[(size_t)gj * m + gi] + acc[bi][bj][e];
+        a.S[(size_t)gj * m + gi] = v;
+        if (gi != gj) a.S[(size_t)gi * m + gj] = v;
+      }
+}
+
+hipError_t launch_schur(const SchurArgs& a, hipStream_t stream) {
+  if (a.m <= 0) return hipSuccess;
+  const int T = a.mpad / NB;
+  hipLaunchKernelGGL(k_schur, dim3(tri(T)), dim3(TPB), 0, stream, a);
+  return hipGetLastError();
+}
+
+// grid (ceil(n / 64), ceil(n / 64)), 256 threads; lane = row (coalesced
+// stores, the transposed reads of U are strided)
+__global__ __launch_bounds__(TPB) void k_block_assemble(BlockAssembleArgs a) {
+  const int i = blockIdx.x * NB + (threadIdx.x & 63);
+  if (i >= a.n) return;
+  for (int r = 0; r < NB / 4; ++r) {
+    const int j = blockIdx.y * NB + (threadIdx.x >> 6) + 4 * r;
+    if (j >= a.n) break;
+    double v = 0.0;
+    if (i >= j) {
+      if (j < a.no)
+        v = (i < a.no) ? a.Loo[(size_t)j * a.ldo + i] : a.U[(size_t)(i - a.no) * a.ldu + j];
+      else
+        v = a.Luu[(size_t)(j - a.no) * a.ldl + (i - a.no)];
+    }
+    a.L[(size_t)j * a.n + i] = v;
+  }
+}
+
+hipError_t launch_block_assemble(const BlockAssembleArgs& a, hipStream_t stream) {
+  if (a.n <= 0) return hipSuccess;
+  const int T = (a.n + NB - 1) / NB;
+  hipLaunchKernelGGL(k_block_assemble, dim3(T, T), dim3(TPB), 0, stream, a);
+  return hipGetLastError();
 }
 
 // One wave per new point j: mean = f_j beta + C(:, j)' z, and

@@ -144,6 +144,7 @@ Runtime::~Runtime() {
     hipFree(kv.second.X);
     hipFree(kv.second.y);
     hipFree(kv.second.F);
+    if (kv.second.owns_M && kv.second.M) hipFree(kv.second.M);
   }
   if (pool_) hipFree(pool_);
   if (d_slots_) hipFree(d_slots_);
@@ -196,6 +197,208 @@ int Runtime::add_problem(const double* X, int n, int d, const double* y, const d
   return id;
 }
 
+int Runtime::add_matrix_problem(const double* M, int n, bool device_memory) {
+  if (n <= 0) throw std::invalid_argument("matrix must have at least one row");
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  DeviceProblem P;
+  P.n = n;
+  P.d = 1;
+  P.p = 0;
+  P.range.assign(1, 0.0);
+  // the likelihood stages that run after the factorisation read X, y: zeros
+  std::vector<double> zeros(n, 0.0);
+  hip_check(hipMalloc(&P.X, sizeof(double) * (size_t)n), "hipMalloc X");
+  hip_check(hipMalloc(&P.y, sizeof(double) * (size_t)n), "hipMalloc y");
+  hip_check(hipMalloc(&P.F, sizeof(double)), "hipMalloc F");
+  hip_check(hipMalloc(&P.M, sizeof(double) * (size_t)n * n), "hipMalloc M");
+  P.owns_M = true;
+  hip_check(hipMemcpy(P.X, zeros.data(), sizeof(double) * n, hipMemcpyHostToDevice), "upload X");
+  hip_check(hipMemcpy(P.y, zeros.data(), sizeof(double) * n, hipMemcpyHostToDevice), "upload y");
+  hip_check(hipMemcpy(P.M, M, sizeof(double) * (size_t)n * n,
+                      device_memory ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice),
+            "upload M");
+  const int id = next_problem_++;
+  problems_[id] = P;
+  return id;
+}
+
+namespace {
+// one evaluation of a matrix problem in the given mode
+EvalResult eval_matrix(Runtime& rt, int pid, int mode) {
+  const double one = 1.0;
+  EvalRequest R;
+  R.problem = pid;
+  R.gamma = &one;
+  R.gamma_n = 1;
+  R.mode = mode;
+  std::vector<EvalRequest> reqs{R};
+  std::vector<EvalResult> res;
+  rt.evaluate(reqs, false, res);
+  return res[0];
+}
+
+// problem ids of matrix problems, removed on scope exit
+struct MatrixProblems {
+  Runtime& rt;
+  std::vector<int> ids;
+  ~MatrixProblems() {
+    for (int id : ids) {
+      try {
+        rt.remove_problem(id);
+      } catch (...) {
+      }
+    }
+  }
+  int add(const double* M, int n, bool device_memory = false) {
+    ids.push_back(rt.add_matrix_problem(M, n, device_memory));
+    return ids.back();
+  }
+};
+}  // namespace
+
+void Runtime::safe_chol(const double* M, int n, double* L, int* inc) {
+  MatrixProblems mp{*this, {}};
+  const int pid = mp.add(M, n);
+  const EvalResult r = eval_matrix(*this, pid, MODE_MATRIX);
+  if (r.status != EVAL_OK) throw std::runtime_error(chol_failure_message(r.status, r.inc, chol.num_nugget));
+  read_slot(0, 0, L);
+  if (inc) *inc = r.inc;
+}
+
+double Runtime::rcond_chol(const double* L, int n) {
+  MatrixProblems mp{*this, {}};
+  const int pid = mp.add(L, n);
+  const EvalResult r = eval_matrix(*this, pid, MODE_FACTOR);
+  return r.status == EVAL_OK ? r.rcond2 : 0.0;
+}
+
+void Runtime::chol_block(const double* C, int n, const double* Loo, int no, double* L) {
+  if (no <= 0 || no > n) throw std::invalid_argument("chol_block: the old block must have 1..n rows");
+  for (int i = 0; i < no; ++i)
+    if (Loo[i + (size_t)i * no] < 0.0)
+      throw std::invalid_argument("chol_block: the old factor must have a non-negative diagonal");
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  MatrixProblems mp{*this, {}};
+  // Loo^-1 from the factor mode (the slot's W, lower, ld = npad)
+  const int pid = mp.add(Loo, no);
+  const EvalResult r = eval_matrix(*this, pid, MODE_FACTOR);
+  if (r.status != EVAL_OK) throw std::runtime_error("chol_block: the old factor has a zero diagonal entry");
+  const SlotView v = slot_view(0);
+  double *dC = nullptr, *dL = nullptr;
+  struct Free {
+    double** p[2];
+    ~Free() {
+      for (double** q : p)
+        if (*q) hipFree(*q);
+    }
+  } fr{{&dC, &dL}};
+  const size_t bn = sizeof(double) * (size_t)n * n;
+  hip_check(hipMalloc(&dC, bn), "hipMalloc");
+  hip_check(hipMalloc(&dL, bn), "hipMalloc");
+  hip_check(hipMemcpy(dC, C, bn, hipMemcpyHostToDevice), "upload C");
+  chol_block_device(dC, n, problem(pid).M, no, v.W, v.npad, no, dL);
+  hip_check(hipMemcpy(L, dL, bn, hipMemcpyDeviceToHost), "read L");
+}
+
+bool Runtime::chol_block_device(const double* dC, int n, const double* dLoo, int ldo, const double* dLooInv, int ldi,
+                                int no, double* dL) {
+  if (no <= 0 || no > n) throw std::invalid_argument("chol_block: the old block must have 1..n rows");
+  if (ldi % NB != 0 || ldi < no) throw std::invalid_argument("chol_block: bad leading dimension of Loo^-1");
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  const int nu = n - no, mpad = (nu + NB - 1) / NB * NB;
+  BlockAssembleArgs ba{};
+  ba.Loo = dLoo;
+  ba.L = dL;
+  ba.n = n;
+  ba.no = no;
+  ba.ldo = ldo;
+  if (nu == 0) {
+    ba.U = dLoo;
+    ba.Luu = dLoo;
+    ba.ldu = ba.ldl = 1;
+    hip_check(launch_block_assemble(ba, stream_), "assemble L");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return true;
+  }
+  double *dB = nullptr, *dU = nullptr, *dCuu = nullptr, *dS = nullptr;
+  struct Free {
+    double** p[4];
+    ~Free() {
+      for (double** q : p)
+        if (*q) hipFree(*q);
+    }
+  } fr{{&dB, &dU, &dCuu, &dS}};
+  const size_t bu = sizeof(double) * (size_t)ldi * mpad, bs = sizeof(double) * (size_t)nu * nu;
+  hip_check(hipMalloc(&dB, bu), "hipMalloc");
+  hip_check(hipMalloc(&dU, bu), "hipMalloc");
+  hip_check(hipMalloc(&dCuu, bs), "hipMalloc");
+  hip_check(hipMalloc(&dS, bs), "hipMalloc");
+  // C_ou (no x nu) into a zero-padded ldi x mpad block, C_uu (nu x nu)
+  hip_check(hipMemsetAsync(dB, 0, bu, stream_), "clear");
+  hip_check(hipMemcpy2DAsync(dB, sizeof(double) * ldi, dC + (size_t)no * n, sizeof(double) * n, sizeof(double) * no,
+                             nu, hipMemcpyDeviceToDevice, stream_),
+            "copy C_ou");
+  hip_check(hipMemcpy2DAsync(dCuu, sizeof(double) * nu, dC + no + (size_t)no * n, sizeof(double) * n,
+                             sizeof(double) * nu, nu, hipMemcpyDeviceToDevice, stream_),
+            "copy C_uu");
+  // U = Loo^-1 C_ou = L_ou' (LinearAlgebra.cpp:271), S = C_uu - U' U
+  PredictArgs t{};
+  t.Linv = dLooInv;
+  t.Ron = dB;
+  t.C = dU;
+  t.npad = ldi;
+  t.mpad = mpad;
+  hip_check(launch_trmm_lower(t, stream_), "trmm");
+  SchurArgs sa{};
+  sa.U = dU;
+  sa.Cuu = dCuu;
+  sa.S = dS;
+  sa.kdim = ldi;
+  sa.m = nu;
+  sa.mpad = mpad;
+  hip_check(launch_schur(sa, stream_), "Schur complement");
+  hip_check(hipStreamSynchronize(stream_), "sync");
+  // L_uu = safe_chol(S); when its ladder fails, safe_chol(C) from scratch
+  // (LinearAlgebra.cpp:286-296)
+  MatrixProblems mp{*this, {}};
+  const int ps = mp.add(dS, nu, true);
+  const EvalResult rs = eval_matrix(*this, ps, MODE_MATRIX);
+  if (rs.status == EVAL_OK) {
+    const SlotView v = slot_view(0);
+    ba.U = dU;
+    ba.ldu = ldi;
+    ba.Luu = v.A;
+    ba.ldl = v.npad;
+    hip_check(launch_block_assemble(ba, stream_), "assemble L");
+    hip_check(hipStreamSynchronize(stream_), "sync");
+    return true;
+  }
+  const int pc = mp.add(dC, n, true);
+  const EvalResult rc = eval_matrix(*this, pc, MODE_MATRIX);
+  if (rc.status != EVAL_OK) throw std::runtime_error(chol_failure_message(rc.status, rc.inc, chol.num_nugget));
+  const SlotView v = slot_view(0);
+  BlockAssembleArgs fa{};
+  fa.L = dL;
+  fa.n = n;
+  fa.no = 0;
+  fa.Loo = v.A;
+  fa.U = v.A;
+  fa.Luu = v.A;
+  fa.ldo = fa.ldu = fa.ldl = v.npad;
+  hip_check(launch_block_assemble(fa, stream_), "assemble L");
+  hip_check(hipStreamSynchronize(stream_), "sync");
+  return false;
+}
+
+void Runtime::set_problem_factor(int id, const double* L) {
+  auto it = problems_.find(id);
+  if (it == problems_.end()) throw std::invalid_argument("unknown problem id");
+  if (it->second.owns_M) throw std::invalid_argument("a matrix problem's matrix is its own");
+  it->second.M = const_cast<double*>(L);
+  // the slot table carries the pointer: lay it out again
+  slot_problem_.clear();
+}
+
 void Runtime::remove_problem(int id) {
   auto it = problems_.find(id);
   if (it == problems_.end()) throw std::invalid_argument("unknown problem id");
@@ -204,6 +407,7 @@ void Runtime::remove_problem(int id) {
   hipFree(it->second.X);
   hipFree(it->second.y);
   hipFree(it->second.F);
+  if (it->second.owns_M && it->second.M) hipFree(it->second.M);
   problems_.erase(it);
   // slots that pointed at it must be rebuilt
   slot_problem_.clear();
@@ -282,6 +486,7 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
     S.X = P.X;
     S.y = P.y;
     S.F = P.F;
+    S.M = P.M;
     S.A = reinterpret_cast<double*>(q);
     q += align256(sizeof(double) * (size_t)npad * npad);
     S.W = reinterpret_cast<double*>(q);
@@ -697,6 +902,8 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
     Q.want_grad = want_grad;
     Q.gamma_has_alpha = R.gamma_n > P.d;
     Q.objective = objective;
+    Q.mode = R.mode;
+    if (R.mode != MODE_KERNEL && P.M == nullptr) throw std::invalid_argument("matrix mode needs a matrix problem");
   }
   for (;;) {
     run_pipeline(want_grad, bench, objective);
@@ -721,6 +928,13 @@ void Runtime::evaluate(const std::vector<EvalRequest>& reqs, bool want_grad, std
         E.rcond2 = O.rcond2;
         E.inc = Q.inc;
         E.status = EVAL_OK;
+        Q.active = 0;
+        continue;
+      }
+      if (Q.mode == MODE_FACTOR) {  // a given factor with a zero (or NaN) diagonal
+        E.status = EVAL_CHOL_FAILED;
+        E.inc = 0;
+        E.rcond2 = 0.0;
         Q.active = 0;
         continue;
       }

@@ -59,6 +59,10 @@ struct EvalRequest {
   // an inactive request keeps its slot (same layout, same captured graph)
   // but is not evaluated; its result is left with ll = NaN
   bool active = true;
+  // MODE_KERNEL (the likelihood), or, for a problem made by
+  // add_matrix_problem, MODE_MATRIX (factor M with the retry ladder) /
+  // MODE_FACTOR (M is the factor)
+  int mode = MODE_KERNEL;
 };
 
 enum EvalStatus { EVAL_OK = 0, EVAL_CHOL_FAILED = 1, EVAL_NUGGET_NOT_POSITIVE = 2 };
@@ -77,6 +81,8 @@ struct DeviceProblem {
   double* X = nullptr;
   double* y = nullptr;
   double* F = nullptr;
+  double* M = nullptr;  // n x n (ld = n) matrix of the matrix / factor modes
+  bool owns_M = false;  // M allocated by add_matrix_problem (else the caller's)
   std::vector<double> range;  // max - min of each input column (prior of the LMP objective)
 };
 
@@ -102,6 +108,32 @@ class Runtime {
   Runtime& operator=(const Runtime&) = delete;
 
   int add_problem(const double* X, int n, int d, const double* y, const double* F, int p, int kernel, int noise);
+  // a problem whose matrix is given (n x n column-major, host or device
+  // memory): evaluated with EvalRequest::mode MODE_MATRIX or MODE_FACTOR
+  int add_matrix_problem(const double* M, int n, bool device_memory = false);
+
+  // LinearAlgebra::safe_chol_lower (LinearAlgebra.cpp:43-98) on the
+  // device: L (n x n, lower) of M + the ladder's jitter and the jitter
+  // steps; throws the reference's messages when the ladder fails
+  void safe_chol(const double* M, int n, double* L, int* inc);
+  // LinearAlgebra::rcond_chol (LinearAlgebra.cpp:106-115): rcond(L)^2 by
+  // the 1-norm estimator, for a given lower factor L
+  double rcond_chol(const double* L, int n);
+  // LinearAlgebra::chol_block (LinearAlgebra.cpp:257-299): the Cholesky
+  // factor of C (n x n) from the factor Loo of its leading no x no block:
+  // L_uo^T = Loo^-1 C_ou, L_uu = safe_chol(C_uu - L_uo L_uo^T), and, when
+  // that fails, safe_chol(C) from scratch
+  void chol_block(const double* C, int n, const double* Loo, int no, double* L);
+  // the same on device buffers: C (n x n, ld = n), Loo (ld = ldo) and its
+  // inverse LooInv (lower, ld = ldi, a multiple of 64 with identity padding,
+  // diagonal blocks zero above the diagonal, as a slot's W) -> L (n x n,
+  // ld = n).  Returns false when it fell back to the full factor of C.
+  bool chol_block_device(const double* C, int n, const double* Loo, int ldo, const double* LooInv, int ldi, int no,
+                         double* L);
+  // the factor a MODE_FACTOR evaluation of problem id takes (n x n, ld = n,
+  // device memory the caller keeps alive until the evaluation returns);
+  // nullptr clears it
+  void set_problem_factor(int id, const double* L);
   void remove_problem(int id);
   const DeviceProblem& problem(int id) const;
 

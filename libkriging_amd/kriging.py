@@ -68,6 +68,11 @@ for _name in ("lk_kriging_is_beta_estim", "lk_kriging_is_theta_estim", "lk_krigi
 N.register("lk_optim_set", ctypes.c_int, [_cp, ctypes.c_double])
 N.register("lk_optim_get", ctypes.c_int, [_cp, _dp])
 N.register("lk_linalg_set", ctypes.c_int, [_cp, ctypes.c_double])
+N.register("lk_linalg_get", ctypes.c_int, [_cp, _dp])
+N.register("lk_linalg_chol_safe", ctypes.c_int, [_dp, ctypes.c_int, _dp, _ip])
+for _name in ("lk_linalg_rcond_chol", "lk_linalg_rcond_approx_chol"):
+    N.register(_name, ctypes.c_int, [_dp, ctypes.c_int, _dp])
+N.register("lk_linalg_chol_block", ctypes.c_int, [_dp, ctypes.c_int, _dp, ctypes.c_int, _dp])
 N.register("lk_random_reset_seed", ctypes.c_int, [ctypes.c_uint])
 LK_OBJECTIVE_FN = ctypes.CFUNCTYPE(ctypes.c_double, _dp, _dp, _vp)
 N.register("lk_lbfgsb_minimize", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, LK_OBJECTIVE_FN, _vp, ctypes.c_int,
@@ -497,6 +502,73 @@ def set_num_nugget(v):
 
 def check_chol_rcond(b):
     N.check(N.lib().lk_linalg_set(b"chol_rcond_check", float(bool(b))))
+
+
+def _linalg_get(name):
+    v = ctypes.c_double()
+    N.check(N.lib().lk_linalg_get(name, ctypes.byref(v)))
+    return v.value
+
+
+def get_num_nugget():
+    """LinearAlgebra::get_num_nugget (LinearAlgebra.cpp:37-39)."""
+    return _linalg_get(b"num_nugget")
+
+
+def chol_rcond_checked():
+    """LinearAlgebra::chol_rcond_checked (LinearAlgebra.cpp:59-61)."""
+    return bool(_linalg_get(b"chol_rcond_check"))
+
+
+def _square(M, name):
+    M = np.asfortranarray(np.asarray(M, dtype=np.float64))
+    if M.ndim != 2 or M.shape[0] != M.shape[1] or M.shape[0] == 0:
+        raise ValueError(f"{name} must be a non-empty square matrix")
+    return M
+
+
+def chol_safe(X, return_inc=False):
+    """LinearAlgebra::safe_chol_lower (LinearAlgebra.cpp:43-98) on the device:
+    the lower factor of X plus the retry ladder's jitter (R binding
+    linalg_chol_safe, bindings/R/rlibkriging/src/linalg.cpp:39-43)."""
+    X = _square(X, "X")
+    n = X.shape[0]
+    L = np.zeros((n, n), order="F")
+    inc = ctypes.c_int()
+    N.check(N.lib().lk_linalg_chol_safe(X.ctypes.data_as(_dp), n, L.ctypes.data_as(_dp), ctypes.byref(inc)))
+    return (L, inc.value) if return_inc else L
+
+
+def rcond_chol(L):
+    """LinearAlgebra::rcond_chol (LinearAlgebra.cpp:106-115): rcond(L)^2 of a
+    lower factor, 1-norm estimate (R binding linalg_rcond_chol)."""
+    L = _square(L, "L")
+    r = ctypes.c_double()
+    N.check(N.lib().lk_linalg_rcond_chol(L.ctypes.data_as(_dp), L.shape[0], ctypes.byref(r)))
+    return r.value
+
+
+def rcond_approx_chol(L):
+    """LinearAlgebra::rcond_approx_chol (LinearAlgebra.cpp:120-139)."""
+    L = _square(L, "L")
+    r = ctypes.c_double()
+    N.check(N.lib().lk_linalg_rcond_approx_chol(L.ctypes.data_as(_dp), L.shape[0], ctypes.byref(r)))
+    return r.value
+
+
+def chol_block(C, Loo):
+    """LinearAlgebra::chol_block (LinearAlgebra.cpp:257-299): the lower factor
+    of C given the factor Loo of its leading block (R binding
+    linalg_chol_block)."""
+    C = _square(C, "C")
+    Loo = _square(Loo, "Loo")
+    n, no = C.shape[0], Loo.shape[0]
+    if no > n:
+        raise ValueError("Loo must not be larger than C")
+    L = np.zeros((n, n), order="F")
+    N.check(N.lib().lk_linalg_chol_block(C.ctypes.data_as(_dp), n, Loo.ctypes.data_as(_dp), no,
+                                         L.ctypes.data_as(_dp)))
+    return L
 
 
 def reset_seed(seed):

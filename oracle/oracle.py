@@ -63,6 +63,8 @@ def lib():
         L.lko_safe_chol_lower.restype = ctypes.c_int
         L.lko_safe_chol_lower.argtypes = [_dp, ctypes.c_int, ctypes.POINTER(CholCfg), _dp, _ip, _dp]
         L.lko_inv_sympd.argtypes = [_dp, ctypes.c_int, _dp]
+        L.lko_solve_lower.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int]
+        L.lko_solve_lower.restype = None
         L.lko_log_likelihood.restype = ctypes.c_int
         L.lko_log_likelihood.argtypes = [ctypes.POINTER(Problem), _dp, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(CholCfg), _dp, _dp, _ip]
@@ -260,6 +262,39 @@ def safe_chol_lower(R, cfg=None):
     if rc != 0:
         raise RuntimeError(lib().lko_last_error().decode())
     return L, inc.value, rc_.value
+
+
+def solve_lower(L, B):
+    """Loo^-1 B by forward substitution (LinearAlgebra::solve on a lower
+    factor, LinearAlgebra.cpp:385-387; LAPACK dtrtrs under use_lapack())."""
+    L = _f64(L)
+    B = _f64(np.array(B, dtype=np.float64).reshape(L.shape[0], -1))
+    lib().lko_solve_lower(_ptr(L), L.shape[0], _ptr(B), B.shape[1])
+    return B
+
+
+def chol_block(C, Loo, cfg=None):
+    """LinearAlgebra::chol_block (LinearAlgebra.cpp:257-299): the lower factor
+    of C from the factor Loo of its leading no x no block.  L_ou' = Loo^-1
+    C_ou, L_uu = safe_chol(C_uu - L_ou L_ou'); when that ladder fails, the
+    factor of C from scratch.  Returns (L, path) with path "block" or
+    "full"."""
+    C = _f64(C)
+    Loo = np.tril(_f64(Loo))
+    n, no = C.shape[0], Loo.shape[0]
+    L = np.zeros((n, n), order="F")
+    L[:no, :no] = Loo
+    if no == n:
+        return L, "block"
+    Lou = solve_lower(Loo, C[:no, no:])
+    L[no:, :no] = Lou.T
+    try:
+        Luu, _, _ = safe_chol_lower(C[no:, no:] - Lou.T @ Lou, cfg)
+    except RuntimeError:
+        Lf, _, _ = safe_chol_lower(C, cfg)
+        return np.tril(Lf), "full"
+    L[no:, no:] = Luu
+    return np.tril(L), "block"
 
 
 def inv_sympd(L):

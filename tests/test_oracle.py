@@ -376,3 +376,56 @@ def test_lapack_route_matches_restated_loops():
         assert info == 0 and rel_err(rc, lapack.dtrcon(np.tril(L), norm="1", uplo="L")[0]) < 1e-12
     finally:
         O.use_lapack(False)
+
+
+def _spd(n, seed, cond=1e3):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    ev = np.logspace(0, -np.log10(cond), n)
+    return (Q * ev) @ Q.T
+
+
+@pytest.mark.parametrize("n,no", [(7, 1), (40, 13), (90, 64), (130, 129)])
+def test_chol_block_equals_the_unique_factor(n, no):
+    # the Cholesky factor with a positive diagonal is unique: given the true
+    # factor of the leading block, the block update must return chol(C)
+    C = _spd(n, 10 + n)
+    Loo = np.linalg.cholesky(C[:no, :no])
+    L, path = O.chol_block(C, Loo)
+    assert path == "block"
+    assert np.allclose(L, np.tril(L))
+    assert np.max(np.abs(L - np.linalg.cholesky(C))) < 1e-12 * np.max(np.abs(L))
+    assert np.max(np.abs(L @ L.T - C)) < 1e-12
+
+
+def test_chol_block_uses_the_given_factor():
+    # a Loo that is not the factor of C_oo enters through L_ou = C_uo Loo^-T
+    # and the Schur complement (scipy's triangular solve and numpy's
+    # Cholesky as the independent route); C_oo itself is never read
+    from scipy.linalg import solve_triangular
+    n, no = 50, 20
+    C = _spd(n, 3)
+    Loo = 2.0 * np.linalg.cholesky(C[:no, :no])
+    L, path = O.chol_block(C, Loo)
+    assert path == "block"
+    Lou = solve_triangular(Loo, C[:no, no:], lower=True)
+    ref = np.zeros((n, n))
+    ref[:no, :no] = Loo
+    ref[no:, :no] = Lou.T
+    ref[no:, no:] = np.linalg.cholesky(C[no:, no:] - Lou.T @ Lou)
+    assert np.max(np.abs(L - ref)) < 1e-12
+    Cb = C.copy()
+    Cb[:no, :no] = 0.0
+    assert np.array_equal(O.chol_block(Cb, Loo)[0], L)
+
+
+def test_chol_block_falls_back_to_the_full_factor():
+    # Loo = chol(C_oo) / 10 makes C_uu - L_ou L_ou' strongly indefinite:
+    # the ladder on it is exhausted (its jitter totals ~1.1) and the result
+    # is the factor of C from scratch (LinearAlgebra.cpp:286-296)
+    n, no = 30, 12
+    C = _spd(n, 5, cond=10.0)
+    Loo = 0.1 * np.linalg.cholesky(C[:no, :no])
+    L, path = O.chol_block(C, Loo)
+    assert path == "full"
+    assert np.max(np.abs(L - np.linalg.cholesky(C))) < 1e-12
