@@ -201,7 +201,8 @@ class Kriging {
   bool est_sigma2_ = true, est_nugget_ = true, est_beta_ = true, est_theta_ = true;
   std::vector<double> beta_;
 
-  // committed model on the device: L (n x n, ld = n), L^-1 (npad x npad),
+  // committed model on the device: L (n x n, ld = n; only the lower
+  // triangle is meaningful), L^-1 (npad x npad),
   // z = E*, M = F*, R*
   double* d_L_ = nullptr;
   double* d_Linv_ = nullptr;

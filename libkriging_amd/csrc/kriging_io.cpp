@@ -53,50 +53,51 @@ double corr(const std::string& kernel, const double* dx, const std::vector<doubl
 
 }  // namespace
 
-// T = L (n x n), M = F* (n x p), z = E* (or y* - M beta when beta is fixed,
-// Kriging.cpp:2170-2172) and, when asked, R^-1, from one device evaluation
-// at the committed parameters that takes the committed factor L as given
-// (the reference keeps m_T, which a block update leaves as chol_block made it)
+// T = L (n x n), M = F* (n x p) and z = E* (or y* - M beta when beta is
+// fixed, Kriging.cpp:2170-2172) of the committed model, as the commit (or a
+// block update) left them on the device (the reference's m_T, m_M, m_z);
+// R^-1 when asked from one device evaluation that takes that L as given
 void Kriging::factors(std::vector<double>* T, std::vector<double>* M, std::vector<double>* z,
                       std::vector<double>* Rinv) const {
   if (is_empty_) throw std::runtime_error("model is not fitted");
+  if (d_L_ == nullptr || d_vec_ == nullptr) throw std::runtime_error("model has no committed factor");
   const int n = n_, p = p_;
-  std::vector<double> g(theta_);
-  if (noise_ == NoiseModel::Nugget) g.push_back(alpha_);
-  std::vector<EvalRequest> reqs{commit_request(g)};
-  reqs[0].flags.alpha = alpha_;
-  std::vector<EvalResult> res;
-  if (d_L_ != nullptr) {
-    rt_->set_problem_factor(problem_, d_L_);
-    reqs[0].mode = MODE_FACTOR;
-  }
-  try {
-    rt_->evaluate(reqs, false, res);
-  } catch (...) {
-    if (d_L_ != nullptr) rt_->set_problem_factor(problem_, nullptr);
-    throw;
-  }
-  if (d_L_ != nullptr) rt_->set_problem_factor(problem_, nullptr);
-  if (res[0].status != EVAL_OK)
-    throw std::runtime_error(chol_failure_message(res[0].status, res[0].inc, rt_->chol.num_nugget));
-  std::vector<double> Mv((size_t)n * std::max(p, 1));
+  hip_check(hipSetDevice(device_), "hipSetDevice");
+  hip_check(hipStreamSynchronize(rt_->stream()), "sync");
   if (T) {
     T->assign((size_t)n * n, 0.0);
-    rt_->read_slot(0, 0, T->data());
+    hip_check(hipMemcpy(T->data(), d_L_, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost), "read T");
+    // the device copy's strict upper triangle is workspace: T is lower
+    for (int j = 1; j < n; ++j)
+      for (int i = 0; i < j; ++i) (*T)[i + (size_t)j * n] = 0.0;
   }
-  if (p > 0) rt_->read_slot(0, 2, Mv.data());
+  if (M) {
+    M->assign((size_t)n * p, 0.0);
+    if (p > 0)
+      hip_check(hipMemcpy2D(M->data(), sizeof(double) * n, d_vec_, sizeof(double) * npad_, sizeof(double) * n, p,
+                            hipMemcpyDeviceToHost),
+                "read M");
+  }
   if (z) {
     z->assign(n, 0.0);
-    if (est_beta_) {
-      rt_->read_slot(0, 4, z->data());
-    } else {
-      rt_->read_slot(0, 3, z->data());
-      for (int i = 0; i < n; ++i)
-        for (int a = 0; a < p; ++a) (*z)[i] -= Mv[i + (size_t)a * n] * beta_[a];
-    }
+    hip_check(hipMemcpy(z->data(), d_vec_ + (size_t)npad_ * p, sizeof(double) * n, hipMemcpyDeviceToHost), "read z");
   }
-  if (M) M->assign(Mv.begin(), Mv.begin() + (size_t)n * p);
   if (Rinv) {
+    std::vector<double> g(theta_);
+    if (noise_ == NoiseModel::Nugget) g.push_back(alpha_);
+    std::vector<EvalRequest> reqs{commit_request(g)};
+    reqs[0].flags.alpha = alpha_;
+    reqs[0].mode = MODE_FACTOR;
+    std::vector<EvalResult> res;
+    rt_->set_problem_factor(problem_, d_L_);
+    try {
+      rt_->evaluate(reqs, false, res);
+    } catch (...) {
+      rt_->set_problem_factor(problem_, nullptr);
+      throw;
+    }
+    rt_->set_problem_factor(problem_, nullptr);
+    if (res[0].status != EVAL_OK) throw std::runtime_error("the committed factor has a zero diagonal entry");
     Rinv->assign((size_t)n * n, 0.0);
     rt_->read_rinv(0, Rinv->data());
   }
