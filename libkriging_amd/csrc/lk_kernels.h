@@ -270,10 +270,14 @@ struct NkArgs {
   double* out;             // 2 qc: mean, variance
   int* info;               // qc: 0 Cholesky, 1 LU fallback, 2 singular
   double theta[DMAX];
+  double itheta[DMAX];     // 1 / theta (1 beyond d): wave-uniform kernel arguments
   double beta0, sigma2, jitter;
   int Npad, d, p, qc, qpad, kernel, npairs;
 };
 hipError_t launch_nk_prepare(const NkArgs& a, hipStream_t stream);  // C, U, M, Kd, R_g^-1 k_g
+// points per workgroup of the pair kernel: the chunk's points are padded to
+// a multiple of it (qpad)
+constexpr int NK_QT = 128;
 hipError_t launch_nk_pairs(const NkArgs& a, hipStream_t stream);    // off-diagonal KM
 hipError_t launch_nk_solve(const NkArgs& a, hipStream_t stream);    // per-point aggregation
 hipError_t launch_nk_alpha(const NkArgs& a, const double* r, hipStream_t stream);  // alpha = Linv' r

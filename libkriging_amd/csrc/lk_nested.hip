@@ -135,54 +135,59 @@ __global__ __launch_bounds__(TPB) void k_nk_alpha(NkArgs a, const double* __rest
 
 // ---------------------------------------------------------------------------
 // (6) KM_ij(t) = sum_a W_i(a, t) sum_b R(x_a, x_b) W_j(b, t) for i < j.
-// grid (npairs, qpad / 64), 256 threads.  Per 64-row tile of group i:
-//   acc(a, t) = sum_b R(a, b) W_j(b, t)    (R chunk generated into LDS,
-//                                            W_j chunk loaded, MFMA)
-//   part(t)  += sum_a W_i(a, t) acc(a, t)  (epilogue, registers)
-// then one fixed-order reduction over the 16 row lanes and the two row waves.
+// grid (npairs, qpad / NK_QT), 256 threads; wave w owns the NK_QW points
+// t0 + NK_QW w .. of the workgroup's NK_QT.  Per 64-row tile of group i and
+// chunk of NK_PKC rows of group j:
+//   R(a, b) generated once into LDS (4 kernel values per thread), then
+//   reused by all NK_QT points on the MFMA
+//   acc(a, t) += sum_b R(a, b) W_j(b, t)   (64 x NK_QW per wave)
+// and per row tile  part(t) += sum_a W_i(a, t) acc(a, t)  in registers.
+// Every sum over rows stays inside one wave (fixed order, no atomics).
 // ---------------------------------------------------------------------------
+constexpr int NK_PKC = 16;      // rows of group j per chunk
+constexpr int NK_QW = NK_QT / 4;  // points per wave
+constexpr int NK_QB = NK_QW / 16;  // 16-point MFMA blocks per wave
+constexpr int NK_SB = NK_QT + 1;  // W_j chunk row stride (odd: transposed stores conflict-free)
+
 template <int DT>
 __global__ __launch_bounds__(TPB, 2) void k_nk_pair(NkArgs a) {
   const int2 pr = a.pairs[blockIdx.x];
-  const int t0 = blockIdx.y * NB;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  const int t0 = blockIdx.y * NK_QT;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r16 = lane & 15, kq = lane >> 4;
   const NkGroup Gi = a.groups[pr.x], Gj = a.groups[pr.y];
   const int d = a.d, ld = a.Npad;
-  // deeper chunks than the plain tile kernels: every generated R value is
-  // reused by 64 points, so fewer, longer chunks amortise the barriers
-  constexpr int PKC = 32;
-  constexpr int WPT = PKC * NB / TPB;  // W_j values per thread per chunk
-  __shared__ double sA[PKC * LDSS];    // R chunk: sA[kk * LDSS + ii] = R(a_ii, b_kk)
-  __shared__ double sB[PKC * LDSS];    // W_j chunk: sB[kk * LDSS + jj] = W_j(b_kk, t0 + jj)
-  __shared__ double sXa[DMAX * NB];    // rows of the current a tile
-  __shared__ double sXb[DMAX * PKC];   // rows of the current b chunk
-  __shared__ double sred[2][NB];
-  double it[DT];
+  constexpr int WV2 = NK_PKC * NK_QT / (2 * TPB);  // W_j 16-byte pairs per thread per chunk
+  constexpr int XR = (DT * NK_PKC + TPB - 1) / TPB;  // X_j values per thread per chunk
+  __shared__ double sA[NK_PKC * LDSS];   // R chunk: sA[kk * LDSS + ii] = R(a_ii, b_kk)
+  __shared__ double sB[NK_PKC * NK_SB];  // W_j chunk: sB[kk * NK_SB + jj] = W_j(b_kk, t0 + jj)
+  __shared__ double sXa[DT * NB];        // rows of the current a tile
+  __shared__ double sXb[DT * NK_PKC];    // rows of the current b chunk
+  const double* it = a.itheta;  // kernel arguments: scalar registers
+  double part[NK_QB][4];
 #pragma unroll
-  for (int k = 0; k < DT; ++k) it[k] = (k < d) ? 1.0 / a.theta[k] : 1.0;
-  double part[2][4];
-#pragma unroll
-  for (int bj = 0; bj < 2; ++bj)
+  for (int bj = 0; bj < NK_QB; ++bj)
 #pragma unroll
     for (int e = 0; e < 4; ++e) part[bj][e] = 0.0;
 
   const double* Wi = a.C + (size_t)t0 * ld + Gi.off;
   const double* Wj = a.C + (size_t)t0 * ld + Gj.off;
   const int ii = tid & 63;
-  // register stage of the next chunk: W_j values and (for tid < d PKC / TPB
-  // rounds) X_j coordinates, loaded while the current chunk is computed
-  double rw[WPT];
-  double rx[(DMAX * PKC) / TPB];
+  // register stage of the next chunk (W_j pairs, X_j coordinates), loaded
+  // while the current chunk is generated and contracted
+  d2 rw[WV2];
+  double rx[XR];
   auto load_chunk = [&](int b0) {
 #pragma unroll
-    for (int r = 0; r < WPT; ++r) {
+    for (int r = 0; r < WV2; ++r) {
       const int q = tid + r * TPB;
-      rw[r] = Wj[(size_t)(q / PKC) * ld + b0 + (q % PKC)];
+      const int bp = q % (NK_PKC / 2), t = q / (NK_PKC / 2);
+      rw[r] = gload(reinterpret_cast<const d2*>(Wj + (size_t)t * ld + b0 + 2 * bp));
     }
 #pragma unroll
-    for (int r = 0; r < (DMAX * PKC) / TPB; ++r) {
+    for (int r = 0; r < XR; ++r) {
       const int q = tid + r * TPB;
-      rx[r] = (q < d * PKC) ? a.Xg[(size_t)(q / PKC) * ld + Gj.off + b0 + (q % PKC)] : 0.0;
+      rx[r] = (q < d * NK_PKC) ? a.Xg[(size_t)(q / NK_PKC) * ld + Gj.off + b0 + (q % NK_PKC)] : 0.0;
     }
   };
   for (int a0 = 0; a0 < Gi.npad; a0 += NB) {
@@ -190,79 +195,83 @@ __global__ __launch_bounds__(TPB, 2) void k_nk_pair(NkArgs a) {
       const int k = q / NB, r = q % NB;
       sXa[q] = a.Xg[(size_t)k * ld + Gi.off + a0 + r];
     }
-    d4 acc[2][2];
-    acc_zero(acc);
-    load_chunk(0);
-    for (int b0 = 0; b0 < Gj.npad; b0 += PKC) {
+    d4 acc[4][NK_QB];
 #pragma unroll
-      for (int r = 0; r < WPT; ++r) {
+    for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < NK_QB; ++bj) acc[bi][bj] = d4{0.0, 0.0, 0.0, 0.0};
+    load_chunk(0);
+    for (int b0 = 0; b0 < Gj.npad; b0 += NK_PKC) {
+#pragma unroll
+      for (int r = 0; r < WV2; ++r) {
         const int q = tid + r * TPB;
-        sB[(q % PKC) * (NB + 1) + q / PKC] = rw[r];  // odd stride: 16 lanes on 16 bank pairs
+        const int bp = q % (NK_PKC / 2), t = q / (NK_PKC / 2);
+        sB[(2 * bp) * NK_SB + t] = rw[r].x;
+        sB[(2 * bp + 1) * NK_SB + t] = rw[r].y;
       }
 #pragma unroll
-      for (int r = 0; r < (DMAX * PKC) / TPB; ++r) {
+      for (int r = 0; r < XR; ++r) {
         const int q = tid + r * TPB;
-        if (q < d * PKC) sXb[q] = rx[r];
+        if (q < d * NK_PKC) sXb[q] = rx[r];
       }
       __syncthreads();
-      if (b0 + PKC < Gj.npad) load_chunk(b0 + PKC);
+      if (b0 + NK_PKC < Gj.npad) load_chunk(b0 + NK_PKC);
       double dx[DT];
 #pragma unroll
-      for (int r = 0; r < (PKC * NB) / TPB; ++r) {
+      for (int r = 0; r < (NK_PKC * NB) / TPB; ++r) {
         const int kk = (tid >> 6) + 4 * r;
 #pragma unroll
-        for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? sXa[k * NB + ii] - sXb[k * PKC + kk] : 0.0;
+        for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? sXa[k * NB + ii] - sXb[k * NK_PKC + kk] : 0.0;
         sA[kk * LDSS + ii] = kern_rho<DT>(a.kernel, dx, it);
       }
       __syncthreads();
-      mma_chunk<false, PKC, LDSS, NB + 1>(sA, sB, acc, lane, wr, wc);
+#pragma unroll
+      for (int k0 = 0; k0 < NK_PKC; k0 += 4) {
+        const double* pa = sA + (k0 + kq) * LDSS + r16;
+        const double* pb = sB + (k0 + kq) * NK_SB + NK_QW * w + r16;
+        double av[4], bv[NK_QB];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) av[q] = pa[16 * q];
+#pragma unroll
+        for (int q = 0; q < NK_QB; ++q) bv[q] = pb[16 * q];
+#pragma unroll
+        for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+          for (int bj = 0; bj < NK_QB; ++bj)
+            acc[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[bj], av[bi], acc[bi][bj], 0, 0, 0);
+      }
       __syncthreads();
     }
-    // epilogue: weight by W_i(a, t) and sum this tile's rows into part
+    // weight by W_i(a, t) and sum this tile's rows into part: lane (r16, kq),
+    // register e of block (bi, bj) holds row 16 bi + r16, point
+    // NK_QW w + 16 bj + kq + 4 e
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < NK_QB; ++bj)
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
+      for (int e = 0; e < 4; ++e) {
+        const double* wcol = Wi + (size_t)(NK_QW * w + 16 * bj + kq + 4 * e) * ld + a0 + r16;
+        double s = part[bj][e];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int row = acc_row(lane, wr, bi), col = acc_col(lane, wc, bj, e);
-          part[bj][e] = fma(acc[bi][bj][e], Wi[(size_t)col * ld + a0 + row], part[bj][e]);
-        }
+        for (int bi = 0; bi < 4; ++bi) s = fma(acc[bi][bj][e], gload(wcol + 16 * bi), s);
+        part[bj][e] = s;
+      }
   }
-  // reduce over the 16 lanes holding different rows of the same columns
+  // reduce over the 16 lanes holding different rows of the same points
+  const size_t pp = (size_t)a.p * a.p;
 #pragma unroll
-  for (int bj = 0; bj < 2; ++bj)
+  for (int bj = 0; bj < NK_QB; ++bj)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       double v = part[bj][e];
 #pragma unroll
       for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
-      part[bj][e] = v;
-    }
-  // waves wr = 0 and wr = 1 hold the two row halves: add them in LDS
-  if (wr == 1 && (lane & 15) == 0) {
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) sred[wc][bj * 16 + (lane >> 4) + 4 * e] = part[bj][e];
-  }
-  __syncthreads();
-  if (wr == 0 && (lane & 15) == 0) {
-    const size_t pp = (size_t)a.p * a.p;
-#pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int col = acc_col(lane, wc, bj, e);
-        const int t = t0 + col;
-        if (t < a.qc) {
-          const double v = part[bj][e] + sred[wc][bj * 16 + (lane >> 4) + 4 * e];
-          double* K = a.KM + (size_t)t * pp;
-          K[(size_t)pr.y * a.p + pr.x] = v;
-          K[(size_t)pr.x * a.p + pr.y] = v;
-        }
+      const int t = t0 + NK_QW * w + 16 * bj + kq + 4 * e;
+      if (r16 == 0 && t < a.qc) {
+        double* K = a.KM + (size_t)t * pp;
+        K[(size_t)pr.y * a.p + pr.x] = v;
+        K[(size_t)pr.x * a.p + pr.y] = v;
       }
-  }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -467,7 +476,8 @@ hipError_t launch_nk_prepare(const NkArgs& a, hipStream_t stream) {
 
 hipError_t launch_nk_pairs(const NkArgs& a, hipStream_t stream) {
   if (a.npairs <= 0) return hipSuccess;
-  const dim3 grid(a.npairs, a.qpad / NB);
+  if (a.qpad % NK_QT != 0) return hipErrorInvalidValue;
+  const dim3 grid(a.npairs, a.qpad / NK_QT);
   if (a.d <= 4)
     hipLaunchKernelGGL(k_nk_pair<4>, grid, dim3(TPB), 0, stream, a);
   else if (a.d <= 8)

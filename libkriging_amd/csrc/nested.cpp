@@ -526,7 +526,7 @@ void NestedKriging::predict_nk(const double* Xn, int m, bool want_stdev, double*
     qc = std::max(64, qc / 64 * 64);
   }
   qc = std::min(qc, m);
-  const int qpad_max = (qc + NB - 1) / NB * NB;
+  const int qpad_max = (qc + NK_QT - 1) / NK_QT * NK_QT;
   double *dXc = nullptr, *dC = nullptr, *dU = nullptr, *dM = nullptr, *dKd = nullptr, *dKM = nullptr, *dKF = nullptr,
          *dout = nullptr;
   int* dinfo = nullptr;
@@ -578,6 +578,7 @@ void NestedKriging::predict_nk(const double* Xn, int m, bool want_stdev, double*
       a.out = dout;
       a.info = dinfo;
       for (int k = 0; k < d; ++k) a.theta[k] = theta_[k];
+      for (int k = 0; k < DMAX; ++k) a.itheta[k] = (k < d) ? 1.0 / theta_[k] : 1.0;
       a.beta0 = beta0_;
       a.sigma2 = sigma2_;
       a.jitter = NK_JITTER;
@@ -585,7 +586,7 @@ void NestedKriging::predict_nk(const double* Xn, int m, bool want_stdev, double*
       a.d = d;
       a.p = p;
       a.qc = mc;
-      a.qpad = (mc + NB - 1) / NB * NB;
+      a.qpad = (mc + NK_QT - 1) / NK_QT * NK_QT;
       a.kernel = kernel_id(kernel_);
       a.npairs = npairs_;
       hip_check(launch_nk_prepare(a, st), "nk prepare");
