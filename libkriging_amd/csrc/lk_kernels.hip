@@ -172,224 +172,279 @@ struct DiagLds {
   double diag[NB], rinv[NB];
 };
 
-// factor sweep for wave W (owns columns 16W .. 16W+15)
+// One step of the factor sweep for wave W (owns columns 16W .. 16W+15),
+// split at its workgroup barrier.  Step p factors columns j0 = DCW p ..
+// j0 + DCW - 1.  Before the barrier the owner of those columns publishes
+// their current values.
 template <int W>
-__device__ inline bool diag_factor(double (&a)[16], DiagLds& sh, int l) {
+__device__ inline void factor_publish(int p, const double (&a)[16], DiagLds& sh, int l) {
+  const int j0 = DCW * p;
+  if (W == j0 / 16) {
+#pragma unroll
+    for (int s = 0; s < DCW; ++s) sh.P[p & 1][s * NB + l] = a[(j0 & 15) + s];
+  }
+}
+
+// After the barrier: the DCW x DCW diagonal sub-block (factored by every lane
+// from LDS broadcasts), this row's L values in the step's columns, the
+// rank-DCW update of the wave's later columns; the owner also writes the
+// step's columns of L (zero above the diagonal) to sh.L, and wave 0 their
+// diagonal and reciprocal, which the inverse sweep reads two steps later.
+template <int W>
+__device__ inline bool factor_step(int p, double (&a)[16], DiagLds& sh, int l) {
   bool ok = true;
+  const int j0 = DCW * p;
+  const double* P = sh.P[p & 1];
+  double* Q = sh.Q[W];
+  // DCW x DCW diagonal sub-block, factored by every lane (left-looking)
+  double L4[DCW][DCW], iv[DCW];
 #pragma unroll
-  for (int p = 0; p < NB / DCW; ++p) {
-    const int j0 = DCW * p;
-    double* P = sh.P[p & 1];
-    double* Q = sh.Q[W];
-    if (W == j0 / 16) {
+  for (int r = 0; r < DCW; ++r)
 #pragma unroll
-      for (int s = 0; s < DCW; ++s) P[s * NB + l] = a[(j0 & 15) + s];
+    for (int c = 0; c <= r; ++c) L4[r][c] = P[c * NB + j0 + r];
+#pragma unroll
+  for (int c = 0; c < DCW; ++c) {
+    double d = L4[c][c];
+#pragma unroll
+    for (int m = 0; m < c; ++m) d = fma(-L4[c][m], L4[c][m], d);
+    ok = ok && (d > 0.0);
+    pivot_sqrt(d, L4[c][c], iv[c]);
+#pragma unroll
+    for (int r = c + 1; r < DCW; ++r) {
+      double t = L4[r][c];
+#pragma unroll
+      for (int m = 0; m < c; ++m) t = fma(-L4[r][m], L4[c][m], t);
+      L4[r][c] = t * iv[c];
     }
-    __syncthreads();
-    // DCW x DCW diagonal sub-block, factored by every lane (left-looking)
-    double L4[DCW][DCW], iv[DCW];
+  }
+  // this row's L values in columns j0 .. j0+DCW-1
+  double x[DCW];
+  {
+    double v[DCW];
 #pragma unroll
-    for (int r = 0; r < DCW; ++r)
+    for (int s = 0; s < DCW; ++s) v[s] = P[s * NB + l];
 #pragma unroll
-      for (int c = 0; c <= r; ++c) L4[r][c] = P[c * NB + j0 + r];
+    for (int s = 0; s < DCW; ++s) {
+      double t = v[s];
 #pragma unroll
-    for (int c = 0; c < DCW; ++c) {
-      double d = L4[c][c];
-#pragma unroll
-      for (int m = 0; m < c; ++m) d = fma(-L4[c][m], L4[c][m], d);
-      ok = ok && (d > 0.0);
-      pivot_sqrt(d, L4[c][c], iv[c]);
-#pragma unroll
-      for (int r = c + 1; r < DCW; ++r) {
-        double t = L4[r][c];
-#pragma unroll
-        for (int m = 0; m < c; ++m) t = fma(-L4[r][m], L4[c][m], t);
-        L4[r][c] = t * iv[c];
-      }
+      for (int m = 0; m < s; ++m) t = fma(-x[m], L4[s][m], t);
+      x[s] = t * iv[s];
     }
-    // this row's L values in columns j0 .. j0+DCW-1
-    double x[DCW];
-    {
-      double v[DCW];
+    // rows inside the block reproduce L4 below the diagonal (same
+    // operations); the diagonal takes the corrected root, rows above the
+    // block are zero.  Entries above the diagonal are left as computed:
+    // they only reach the strict upper part of this tile's columns, which
+    // no lower-part value reads and the stores zero.
 #pragma unroll
-      for (int s = 0; s < DCW; ++s) v[s] = P[s * NB + l];
-#pragma unroll
-      for (int s = 0; s < DCW; ++s) {
-        double t = v[s];
-#pragma unroll
-        for (int m = 0; m < s; ++m) t = fma(-x[m], L4[s][m], t);
-        x[s] = t * iv[s];
-      }
-      // rows inside the block reproduce L4 below the diagonal (same
-      // operations); the diagonal takes the corrected root, rows above the
-      // block are zero.  Entries above the diagonal are left as computed:
-      // they only reach the strict upper part of this tile's columns, which
-      // no lower-part value reads and the store zeroes.
-#pragma unroll
-      for (int s = 0; s < DCW; ++s) {
-        x[s] = (l == j0 + s) ? L4[s][s] : x[s];
-        x[s] = (l < j0) ? 0.0 : x[s];
-      }
+    for (int s = 0; s < DCW; ++s) {
+      x[s] = (l == j0 + s) ? L4[s][s] : x[s];
+      x[s] = (l < j0) ? 0.0 : x[s];
     }
-    // every wave computed the same x for all 64 rows: each keeps its own copy
-    // and reads its columns' values back without a workgroup barrier (one
-    // wave's LDS operations complete in order; the fences keep the compiler
-    // from moving the reads above the writes)
-    if (W == j0 / 16) {
+  }
+  // every wave computed the same x for all 64 rows: each keeps its own copy
+  // and reads its columns' values back without a workgroup barrier (one
+  // wave's LDS operations complete in order; the fences keep the compiler
+  // from moving the reads above the writes)
+  if (W == j0 / 16) {
 #pragma unroll
-      for (int s = 0; s < DCW; ++s) a[(j0 & 15) + s] = x[s];
+    for (int s = 0; s < DCW; ++s) {
+      a[(j0 & 15) + s] = x[s];
+      sh.L[(j0 + s) * (NB + 2) + l] = (l >= j0 + s) ? x[s] : 0.0;
     }
+  }
 #pragma unroll
-    for (int s = 0; s < DCW; ++s) Q[s * NB + l] = x[s];
-    if (W == 0 && l == 0) {
+  for (int s = 0; s < DCW; ++s) Q[s * NB + l] = x[s];
+  if (W == 0 && l == 0) {
 #pragma unroll
-      for (int s = 0; s < DCW; ++s) {
-        sh.diag[j0 + s] = L4[s][s];
-        sh.rinv[j0 + s] = iv[s];
-      }
+    for (int s = 0; s < DCW; ++s) {
+      sh.diag[j0 + s] = L4[s][s];
+      sh.rinv[j0 + s] = iv[s];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // rank-DCW update of this wave's columns right of the block
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // rank-DCW update of this wave's columns right of the block
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int c = 16 * W + q;
-      if (c >= j0 + DCW) {
-        double t = a[q];
+  for (int q = 0; q < 16; ++q) {
+    const int c = 16 * W + q;
+    if (c >= j0 + DCW) {
+      double t = a[q];
 #pragma unroll
-        for (int s = 0; s < DCW; ++s) t = fma(-x[s], Q[s * NB + c], t);
-        a[q] = t;
-      }
+      for (int s = 0; s < DCW; ++s) t = fma(-x[s], Q[s * NB + c], t);
+      a[q] = t;
     }
   }
   return ok;
 }
 
-// inverse sweep for wave W (owns rows 16W .. 16W+15 of X = L_kk^-1; lane = column)
+// One step of the inverse sweep for wave W (owns rows 16W .. 16W+15 of
+// X = L_kk^-1; lane = column), split at its barrier.  Step p solves rows
+// j0 = DCW p .. j0 + DCW - 1 (owner) after reading the wave's L(i, j0 ..)
+// for the update that follows the barrier.
 template <int W>
-__device__ inline void diag_inverse(DiagLds& sh, int l) {
-  double acc[16];
+__device__ inline void inverse_solve(int p, const double (&acc)[16], double (&Lu)[16][DCW], DiagLds& sh, int l) {
+  const int j0 = DCW * p;
+  double* Rr = sh.R[p & 1];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+  for (int q = 0; q < 16; ++q)
+    if (16 * W + q >= j0 + DCW)
 #pragma unroll
-  for (int p = 0; p < NB / DCW; ++p) {
-    const int j0 = DCW * p;
-    double* Rr = sh.R[p & 1];
-    // this wave's L(i, j0 .. j0+DCW-1) for the update below, read before the
-    // barrier (L is read-only here): the loads overlap the owner's chain
-    // instead of following the barrier
-    double Lu[16][DCW];
+      for (int s = 0; s < DCW; ++s) Lu[q][s] = sh.L[(j0 + s) * (NB + 2) + 16 * W + q];
+  if (W == j0 / 16) {
+    double x[DCW];
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
-      if (16 * W + q >= j0 + DCW)
+    for (int s = 0; s < DCW; ++s) {
+      double t = ((l == j0 + s) ? 1.0 : 0.0) - acc[(j0 & 15) + s];
 #pragma unroll
-        for (int s = 0; s < DCW; ++s) Lu[q][s] = sh.L[(j0 + s) * (NB + 2) + 16 * W + q];
-    if (W == j0 / 16) {
-      double x[DCW];
-#pragma unroll
-      for (int s = 0; s < DCW; ++s) {
-        double t = ((l == j0 + s) ? 1.0 : 0.0) - acc[(j0 & 15) + s];
-#pragma unroll
-        for (int m = 0; m < s; ++m) t = fma(-sh.L[(j0 + m) * (NB + 2) + j0 + s], x[m], t);
-        x[s] = t * sh.rinv[j0 + s];
-        Rr[s * NB + l] = x[s];
-        sh.X[(j0 + s) * (NB + 1) + l] = x[s];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int i = 16 * W + q;
-      if (i >= j0 + DCW) {
-        double t = acc[q];
-#pragma unroll
-        for (int s = 0; s < DCW; ++s) t = fma(Lu[q][s], Rr[s * NB + l], t);
-        acc[q] = t;
-      }
+      for (int m = 0; m < s; ++m) t = fma(-sh.L[(j0 + m) * (NB + 2) + j0 + s], x[m], t);
+      x[s] = t * sh.rinv[j0 + s];
+      Rr[s * NB + l] = x[s];
+      sh.X[(j0 + s) * (NB + 1) + l] = x[s];
     }
   }
 }
 
-__global__ __launch_bounds__(TPB) void k_potrf_diag(const Slot* __restrict__ slots,
-                                                    const SlotParams* __restrict__ params,
-                                                    SlotOut* __restrict__ out, int k) {
+template <int W>
+__device__ inline void inverse_update(int p, double (&acc)[16], const double (&Lu)[16][DCW], const DiagLds& sh,
+                                      int l) {
+  const int j0 = DCW * p;
+  const double* Rr = sh.R[p & 1];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = 16 * W + q;
+    if (i >= j0 + DCW) {
+      double t = acc[q];
+#pragma unroll
+      for (int s = 0; s < DCW; ++s) t = fma(Lu[q][s], Rr[s * NB + l], t);
+      acc[q] = t;
+    }
+  }
+}
+
+// The two sweeps pipelined over the workgroup's 8 waves: waves 0-3 run the
+// factor sweep, waves 4-7 the inverse sweep two steps behind it (step p of
+// the inverse reads the L columns and reciprocals that factor step p wrote,
+// one barrier earlier).  Every wave passes the same NB / DCW + 2 barriers.
+// With the factor given (factor mode), sh.L, sh.diag and sh.rinv are filled
+// before the sweep and the factor waves only keep the barrier count.
+template <int W8>
+__device__ inline bool diag_sweeps(double (&a)[16], DiagLds& sh, int l, bool factor_given) {
+  constexpr bool FW = W8 < 4;
+  constexpr int W = FW ? W8 : W8 - 4;
+  constexpr int NR = NB / DCW;
+  bool ok = true;
+  double acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+#pragma unroll
+  for (int r = 0; r < NR + 2; ++r) {
+    double Lu[16][DCW];
+    if (FW) {
+      if (r < NR && !factor_given) factor_publish<W>(r, a, sh, l);
+    } else if (r >= 2) {
+      inverse_solve<W>(r - 2, acc, Lu, sh, l);
+    }
+    __syncthreads();
+    if (FW) {
+      if (r < NR && !factor_given) ok = factor_step<W>(r, a, sh, l) && ok;
+    } else if (r >= 2) {
+      inverse_update<W>(r - 2, acc, Lu, sh, l);
+    }
+  }
+  return ok;
+}
+
+constexpr int DIAG_TPB = 2 * TPB;  // 8 waves: 4 factor, 4 inverse
+
+__global__ __launch_bounds__(DIAG_TPB) void k_potrf_diag(const Slot* __restrict__ slots,
+                                                         const SlotParams* __restrict__ params,
+                                                         SlotOut* __restrict__ out, int k) {
   const int b = blockIdx.x;
   const Slot S = slots[b];
   if (!slot_live(params[b], out[b]) || k >= S.nblk) return;
   const int tid = threadIdx.x, l = tid & 63, w = tid >> 6;
+  const bool fw = w < 4;  // factor wave (else inverse wave w - 4)
   const int k0 = k * NB;
   const int ld = S.npad;
   double* Ab = S.A + (size_t)k0 * ld + k0;
   double* Wb = S.W + (size_t)k0 * ld + k0;
   __shared__ DiagLds sh;
   __shared__ int sbad;
+  __shared__ double smax[2][4];
   if (tid == 0) sbad = 0;
+  const bool given = params[b].mode == MODE_FACTOR;
 
   double a[16];
+  if (fw) {
 #pragma unroll
-  for (int q = 0; q < 16; ++q) a[q] = Ab[(size_t)(16 * w + q) * ld + l];
-  bool ok;
-  if (params[b].mode == MODE_FACTOR) {
-    // the block is already L_kk: its diagonal and reciprocals for the
-    // inverse sweep; a non-positive or NaN diagonal entry is a failure
-    if (w == (l >> 4)) {
-      sh.diag[l] = a[l & 15];
-      sh.rinv[l] = 1.0 / a[l & 15];
-    }
-    __syncthreads();
-    ok = true;
-    for (int q = 0; q < NB; ++q) ok = ok && (sh.diag[q] > 0.0);
-  } else {
-    switch (w) {
-      case 0: ok = diag_factor<0>(a, sh, l); break;
-      case 1: ok = diag_factor<1>(a, sh, l); break;
-      case 2: ok = diag_factor<2>(a, sh, l); break;
-      default: ok = diag_factor<3>(a, sh, l); break;
+    for (int q = 0; q < 16; ++q) a[q] = Ab[(size_t)(16 * w + q) * ld + l];
+    if (given) {
+      // the block is already L_kk: its columns, diagonal and reciprocals for
+      // the inverse sweep; a non-positive or NaN diagonal entry is a failure
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int c = 16 * w + q;
+        sh.L[c * (NB + 2) + l] = (l >= c) ? a[q] : 0.0;
+        if (l == c) {
+          sh.diag[c] = a[q];
+          sh.rinv[c] = 1.0 / a[q];
+        }
+      }
     }
   }
-  // every lane saw every pivot, so ok is uniform; no early exit before the
-  // sweeps' barriers
-  if (!ok) {
+  __syncthreads();
+  bool ok;
+  switch (w) {
+    case 0: ok = diag_sweeps<0>(a, sh, l, given); break;
+    case 1: ok = diag_sweeps<1>(a, sh, l, given); break;
+    case 2: ok = diag_sweeps<2>(a, sh, l, given); break;
+    case 3: ok = diag_sweeps<3>(a, sh, l, given); break;
+    case 4: ok = diag_sweeps<4>(a, sh, l, given); break;
+    case 5: ok = diag_sweeps<5>(a, sh, l, given); break;
+    case 6: ok = diag_sweeps<6>(a, sh, l, given); break;
+    default: ok = diag_sweeps<7>(a, sh, l, given); break;
+  }
+  if (given && fw && l < 16) ok = sh.diag[16 * w + l] > 0.0;
+  // every factor lane saw every pivot; in factor mode a lane checks one
+  // diagonal entry: any failure is shared through LDS before the uniform exit
+  if (fw && !ok) sbad = 1;
+  __syncthreads();
+  if (sbad) {
     if (tid == 0) out[b].status = ST_CHOL_FAILED;
     return;
   }
-  // L_kk -> A (lower, zero above) and LDS
-  double lmax = 0.0;
+  if (fw) {
+    // L_kk -> A (lower, zero above)
+    double lmax = 0.0;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int c = 16 * w + q;
-    const double v = (l >= c) ? a[q] : 0.0;
-    Ab[(size_t)c * ld + l] = v;
-    sh.L[c * (NB + 2) + l] = v;
-    lmax = fmax(lmax, fabs(v));
-  }
-  __shared__ double smax[2][4];
-  lmax = wave_max(lmax);
-  if (l == 0) smax[0][w] = lmax;
-  __syncthreads();
-  if (w == 0) {
-    const double ld_part = wave_sum(log(sh.diag[l]));
-    if (l == 0) S.red[RED_LOGDET + k] = ld_part;
-  }
-  switch (w) {
-    case 0: diag_inverse<0>(sh, l); break;
-    case 1: diag_inverse<1>(sh, l); break;
-    case 2: diag_inverse<2>(sh, l); break;
-    default: diag_inverse<3>(sh, l); break;
-  }
-  __syncthreads();
-  // X rows staged as sh.X[row][col]; write columns coalesced (lane = row)
-  double xmax = 0.0;
+    for (int q = 0; q < 16; ++q) {
+      const int c = 16 * w + q;
+      const double v = (l >= c) ? a[q] : 0.0;
+      Ab[(size_t)c * ld + l] = v;
+      lmax = fmax(lmax, fabs(v));
+    }
+    lmax = wave_max(lmax);
+    if (l == 0) smax[0][w] = lmax;
+    if (w == 0) {
+      const double ld_part = wave_sum(log(sh.diag[l]));
+      if (l == 0) S.red[RED_LOGDET + k] = ld_part;
+    }
+  } else {
+    // L_kk^-1 -> W: rows staged as sh.X[row][col]; columns written coalesced
+    // (lane = row)
+    const int wi = w - 4;
+    double xmax = 0.0;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int c = 16 * w + q;
-    const double v = sh.X[l * (NB + 1) + c];
-    Wb[(size_t)c * ld + l] = v;
-    xmax = fmax(xmax, fabs(v));
+    for (int q = 0; q < 16; ++q) {
+      const int c = 16 * wi + q;
+      const double v = sh.X[l * (NB + 1) + c];
+      Wb[(size_t)c * ld + l] = v;
+      xmax = fmax(xmax, fabs(v));
+    }
+    xmax = wave_max(xmax);
+    if (l == 0) smax[1][wi] = xmax;
   }
-  xmax = wave_max(xmax);
-  if (l == 0) smax[1][w] = xmax;
   __syncthreads();
   // condition estimate of L_kk for the panel kernel's refinement decision
   if (tid == 0) {
@@ -397,7 +452,6 @@ __global__ __launch_bounds__(TPB) void k_potrf_diag(const Slot* __restrict__ slo
     const double xm = fmax(fmax(smax[1][0], smax[1][1]), fmax(smax[1][2], smax[1][3]));
     S.red[red_cond_off(S.nblk) + k] = (double)NB * lm * xm;
   }
-  (void)sbad;
 }
 
 // ---------------------------------------------------------------------------
@@ -1596,7 +1650,7 @@ hipError_t launch_build(const Batch& B) {
   return hipGetLastError();
 }
 hipError_t launch_potrf_diag(const Batch& B, int k) {
-  hipLaunchKernelGGL(k_potrf_diag, dim3(B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, k);
+  hipLaunchKernelGGL(k_potrf_diag, dim3(B.nslots), dim3(DIAG_TPB), 0, B.stream, B.slots, B.params, B.out, k);
   return hipGetLastError();
 }
 hipError_t launch_potrf_panel(const Batch& B, int k) {

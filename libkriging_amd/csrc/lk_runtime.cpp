@@ -43,6 +43,7 @@ Runtime::Runtime(int device) : device_(device) {
     la_workgroups_ = 2 * std::max(1, cus);
   }
   if (const char* e = std::getenv("LKGPU_LOOKAHEAD")) lookahead_ = std::atoi(e) != 0;
+  if (const char* e = std::getenv("LKGPU_LOOKAHEAD_WG")) la_workgroups_ = std::max(0, std::atoi(e));
   // the allocation-free entry point's scratch: its device address, looked
   // up once per device with this device current
   hip_check(noalloc_scratch(device, &scratch_), "hipGetSymbolAddress(scratch)");
@@ -586,8 +587,12 @@ std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_gra
       if (K2 < B.nblk_max) {
         v.push_back({1u << 1, "record chain", [sa, evA] { return hipEventRecord(evA, sa); }});
         v.push_back({1u << 1, "fork bulk update", [sl, evA] { return hipStreamWaitEvent(sl, evA, 0); }});
-        v.push_back({1u << 1, "potrf_update (bulk, persistent)",
-                     [BL, K0, K1, K2, wg] { return launch_potrf_update_persistent(BL, K0, K1, K2, -1, wg); }});
+        if (wg > 0)
+          v.push_back({1u << 1, "potrf_update (bulk, persistent)",
+                       [BL, K0, K1, K2, wg] { return launch_potrf_update_persistent(BL, K0, K1, K2, -1, wg); }});
+        else
+          v.push_back({1u << 1, "potrf_update (bulk)",
+                       [BL, K0, K1, K2] { return launch_potrf_update(BL, K0, K1, K2, -1); }});
         v.push_back({1u << 1, "record bulk update", [sl, evB] { return hipEventRecord(evB, sl); }});
         pending = true;
       }
