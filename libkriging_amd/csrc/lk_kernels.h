@@ -112,7 +112,11 @@ __host__ __device__ inline int red_rstar_off(int nblk) { return 4 * nblk + nblk 
 // cond(L_kk) up to the factor 64 (written by the diagonal-block kernel, read
 // by the panel kernel)
 __host__ __device__ inline int red_cond_off(int nblk) { return red_rstar_off(nblk) + PMAX * PMAX; }
-__host__ __device__ inline size_t red_size(int nblk) { return (size_t)red_cond_off(nblk) + nblk; }
+// per row block r >= 1 and column c of the blocks left of it: sum_i |L(i, c)|
+// over the block's rows, written by the panel kernel (r * npad + c), read
+// for ||L||_1 by k_solve_x
+__host__ __device__ inline int red_lcol_off(int nblk) { return red_cond_off(nblk) + nblk; }
+__host__ __device__ inline size_t red_size(int nblk) { return (size_t)red_lcol_off(nblk) + (size_t)nblk * nblk * NB; }
 // Panels whose diagonal block's condition estimate exceeds this get one step
 // of iterative refinement: multiplying by the explicit inverse of an
 // ill-conditioned block is not backward stable, the refined product is.

@@ -494,6 +494,36 @@ __global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ sl
     tile_mainloop<false, false, false>(S.A, ld, i0, S.W, ld, k0, k0, k0 + NB, sbuf, acc, tid, lane, wr, wc);
   }
   acc_store(acc, A21, ld, lane, wr, wc);
+  // this block's column sums of |L| for ||L||_1: rows bi in the lane, the 16
+  // lanes of a column group by shuffles, the two row waves through LDS
+  __shared__ double scol[NB];
+  double cs[2][4];
+#pragma unroll
+  for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double v = fabs(acc[0][bj][e]) + fabs(acc[1][bj][e]);
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 64);
+      cs[bj][e] = v;
+    }
+  if (wr == 1 && (lane & 15) == 0) {
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) scol[acc_col(lane, wc, bj, e)] = cs[bj][e];
+  }
+  __syncthreads();
+  if (wr == 0 && (lane & 15) == 0) {
+    double* part = S.red + red_lcol_off(S.nblk) + (size_t)r * ld + k0;
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int c = acc_col(lane, wc, bj, e);
+        part[c] = cs[bj][e] + scol[c];
+      }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -735,7 +765,12 @@ __global__ __launch_bounds__(MV_TPB) void k_solve_x(const Slot* __restrict__ slo
   const double* e = S.V + (size_t)(S.p + 2) * ld;
   double* x = S.V + (size_t)(S.p + 3) * ld;
   double mW = 0.0, mL = 0.0;
-  constexpr int XU = 4;  // 3 XU independent loads per lane per batch
+  // ||L||_1: L's columns are read only inside the diagonal block; below it
+  // the panel kernel's per-block column sums stand in for them (a factor
+  // given as input had no panel step: then all of L is read)
+  const int lend = (params[b].mode == MODE_FACTOR) ? ld : (ts.tile + 1) * NB;
+  const double* lpart = S.red + red_lcol_off(S.nblk);
+  constexpr int XU = 4;  // 2 XU independent loads per lane per batch
   for (int cc = w; cc < NB; cc += MV_W) {
     const int c = ts.tile * NB + cc;
     const double* __restrict__ Wc = S.W + (size_t)c * ld;
@@ -743,22 +778,23 @@ __global__ __launch_bounds__(MV_TPB) void k_solve_x(const Slot* __restrict__ slo
     double s = 0.0, aw = 0.0, al = 0.0;
     for (int b0 = c; b0 < ld; b0 += 64 * XU) {
       const int ib = b0 + lane;
-      double wv[XU], ev[XU], av[XU];
+      double wv[XU], ev[XU];
 #pragma unroll
       for (int u = 0; u < XU; ++u) {
         const int i = min(ib + 64 * u, ld - 1);
         wv[u] = gload(Wc + i);
         ev[u] = gload(e + i);
-        av[u] = gload(Ac + i);
       }
 #pragma unroll
       for (int u = 0; u < XU; ++u)
         if (ib + 64 * u < ld) {
           s = fma(wv[u], ev[u], s);
           aw += fabs(wv[u]);
-          al += fabs(av[u]);
         }
     }
+    for (int i = c + lane; i < lend; i += 64) al += fabs(gload(Ac + i));
+    if (lend < ld)
+      for (int r = ts.tile + 1 + lane; r < S.nblk; r += 64) al += lpart[(size_t)r * ld + c];
     s = wave_sum(s);
     aw = wave_sum(aw);
     al = wave_sum(al);
