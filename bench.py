@@ -279,8 +279,8 @@ def main():
     pid = ctx.problem(X, y, np.ones((n, 1)), "exp")
     ids = [pid] * NSTART
 
-    # batched path: one hipGraph replay per step, the batch split over the
-    # context's streams (4 parts of 16 starts)
+    # batched path: one hipGraph replay per step, the batch split into the
+    # parts (one stream each) the launch table gives this size bucket
     for _ in range(args.warmup):
         ctx.ll_grad_batch(ids, theta, True)
     ctx.synchronize()
@@ -289,6 +289,7 @@ def main():
         ll, grad, st, inc = ctx.ll_grad_batch(ids, theta, True)
     ctx.synchronize()
     t_batched = time.perf_counter() - t0
+    parts = ctx.last_parts()
     assert (st == 0).all(), "an evaluation failed"
     evals_s = NSTART * args.steps / t_batched
 
@@ -358,7 +359,7 @@ def main():
         "plain_evals_per_s": plain_evals_s,
         "batched_vs_plain": evals_s / plain_evals_s,
         "plain_equals_batched": same,
-        "streams": 4,
+        "streams": parts,
         "split_equals_single_stream": split_equals_single,
         "roofline": roof,
     }
