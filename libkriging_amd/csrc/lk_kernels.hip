@@ -914,7 +914,7 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
 // grid (nblk_max (nblk_max + 1) / 2, nslots), 256 threads
 // ---------------------------------------------------------------------------
 template <int DT, bool LMP>
-__global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ slots,
+__global__ __launch_bounds__(TPB, (DT == 16 ? 2 : 1)) void k_lauum_adjoint(const Slot* __restrict__ slots,
                                                        const SlotParams* __restrict__ params,
                                                        SlotOut* __restrict__ out) {
   const TileSlot ts = tile_slot();
@@ -964,11 +964,19 @@ __global__ __launch_bounds__(TPB) void k_lauum_adjoint(const Slot* __restrict__ 
       sG[PMAX * NB + q] = Gt[(size_t)a * ld + j0 + r];
     }
   }
-  __syncthreads();
-
-  double it_[DT];
+  // 1 / theta_k: in registers up to 8 dimensions; beyond, read from LDS
+  // (broadcast) so that the epilogue's registers leave room for more waves
+  constexpr bool kItLds = DT > 8;
+  __shared__ double sIt[kItLds ? DT : 1];
+  double itr[kItLds ? 1 : DT];
+  if (kItLds) {
+    if (tid < DT) sIt[tid] = 1.0 / ((tid < d) ? P.theta[tid] : 1.0);
+  } else {
 #pragma unroll
-  for (int k = 0; k < DT; ++k) it_[k] = 1.0 / ((k < d) ? P.theta[k] : 1.0);
+    for (int k = 0; k < (kItLds ? 1 : DT); ++k) itr[k] = 1.0 / ((k < d) ? P.theta[k] : 1.0);
+  }
+  __syncthreads();
+  const double* it_ = kItLds ? sIt : itr;
   const double factor = (S.noise == NOISE_NUGGET) ? P.alpha : 1.0;
   const int kern = S.kernel;
   // one sum per theta_k: sum (x_i x_j - sigma2 Rinv_ij) dR_ij/dtheta_k, the
@@ -1058,14 +1066,33 @@ __global__ __launch_bounds__(TPB) void k_finalize_grad(const Slot* __restrict__ 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int ntiles = S.nblk * (S.nblk + 1) / 2;
   __shared__ double tot[GRAD_STRIDE];
-  // wave w sums components w, w + 4, ...; lanes stride over tiles, then a
-  // fixed-order butterfly: deterministic for a given tile count
-  for (int q = w; q < GRAD_STRIDE; q += TPB / 64) {
-    const double* src = S.red + red_grad_off(S.nblk) + q;
-    double s = 0.0;
-    for (int t = lane; t < ntiles; t += 64) s += src[(size_t)t * GRAD_STRIDE];
-    s = wave_sum(s);
-    if (lane == 0) tot[q] = s;
+  __shared__ double part[TPB / 64][GRAD_STRIDE];
+  // wave w sums the tiles of its quarter, lane = component (a tile's
+  // partials are contiguous: coalesced rows), four running sums in tile
+  // order; the quarters are added in order: deterministic for a given tile
+  // count
+  {
+    const double* src = S.red + red_grad_off(S.nblk);
+    const int t0 = (int)(((long long)ntiles * w) / (TPB / 64));
+    const int t1 = (int)(((long long)ntiles * (w + 1)) / (TPB / 64));
+    for (int q = lane; q < GRAD_STRIDE; q += 64) {
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int t = t0;
+      for (; t + 3 < t1; t += 4) {
+        s0 += src[(size_t)t * GRAD_STRIDE + q];
+        s1 += src[(size_t)(t + 1) * GRAD_STRIDE + q];
+        s2 += src[(size_t)(t + 2) * GRAD_STRIDE + q];
+        s3 += src[(size_t)(t + 3) * GRAD_STRIDE + q];
+      }
+      for (; t < t1; ++t) s0 += src[(size_t)t * GRAD_STRIDE + q];
+      part[w][q] = (s0 + s1) + (s2 + s3);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < GRAD_STRIDE; q += TPB) {
+    double s = part[0][q];
+    for (int v = 1; v < TPB / 64; ++v) s += part[v][q];
+    tot[q] = s;
   }
   __syncthreads();
   const int q = threadIdx.x;
