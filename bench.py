@@ -189,6 +189,46 @@ def nested_workload(seed=127, n=200_000, d=5, groups=256):
     return X, y, group_of, sizes, Xn
 
 
+def bucket_fractions(X, y, group_of, theta, reps=5):
+    """LL+grad of the NestedKriging groups at theta, batched by size bucket
+    (64-row blocks per group: 5..8, 9..16, 17..24) and all 256 groups in one
+    uneven batch: evaluations/s and the FP64 MFMA fraction of n_g^3 flops
+    per evaluation (n^3 / 3 each for the Cholesky, L^-1 and the R^-1 tiles)."""
+    from libkriging_amd import _native as N
+    ctx = N.Context(0)
+    ng = np.bincount(group_of)
+    pids = []
+    for g in range(len(ng)):
+        idx = np.where(group_of == g)[0]
+        pids.append(ctx.problem(X[idx], y[idx], np.ones((len(idx), 1)), "matern3_2"))
+    nblk = (ng + 63) // 64
+    sets = {"nblk 5-8": [g for g in range(len(ng)) if nblk[g] <= 8],
+            "nblk 9-16": [g for g in range(len(ng)) if 9 <= nblk[g] <= 16],
+            "nblk 17-32": [g for g in range(len(ng)) if 17 <= nblk[g] <= 32],
+            "all groups": list(range(len(ng)))}
+    out = {}
+    for name, sel in sets.items():
+        if not sel:
+            continue
+        ids = [pids[g] for g in sel]
+        th = np.repeat(np.asarray(theta, dtype=np.float64)[None, :], len(sel), axis=0)
+        for _ in range(2):
+            ctx.ll_grad_batch(ids, th, True)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.ll_grad_batch(ids, th, True)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        tf = float(np.sum(ng[sel].astype(np.float64) ** 3)) / dt / 1e12
+        out[name] = {"groups": len(sel), "ms": 1e3 * dt, "evals_per_s": len(sel) / dt, "tflops": tf,
+                     "frac_of_fp64_mfma_peak": tf / MFMA_FP64_PEAK_TFLOPS, "parts": ctx.last_parts()}
+    for p in pids:
+        ctx.drop_problem(p)
+    ctx.close()
+    return out
+
+
 def nested_leg(points=2000):
     """The uneven NestedKriging configuration: fit (all 256 groups' BFGS in
     shared batched pipelines) and NK predict on `points` of the 10 000
@@ -209,6 +249,7 @@ def nested_leg(points=2000):
     t_pred = time.perf_counter() - t0
     pair_ms, pair_launches, pair_points = nk.pair_time()
     nk.time_pairs(False)
+    buckets = bucket_fractions(X, y, group_of, nk.theta())
     s = sizes.astype(np.float64)
     pair_flops = (s.sum() ** 2 - np.sum(s ** 2)) * pair_points
     achieved = pair_flops / (pair_ms * 1e-3) / 1e12
@@ -221,6 +262,7 @@ def nested_leg(points=2000):
         "nk_pair": {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": achieved / MFMA_FP64_PEAK_TFLOPS, "kernel": "k_nk_pair", "ms": pair_ms,
                     "launches": pair_launches},
+        "ll_grad_by_bucket": buckets,
         "theta": nk.theta().tolist(), "sigma2": nk.sigma2(), "beta0": nk.beta0(),
         "finite": bool(np.all(np.isfinite(mean)) and np.all(np.isfinite(sd))),
     }

@@ -82,8 +82,8 @@ __global__ __launch_bounds__(TPB) void k_build(const Slot* __restrict__ slots, c
     return;
   }
 
-  __shared__ double sXi[DMAX * NB];
-  __shared__ double sXj[DMAX * NB];
+  __shared__ double sXi[DT * NB];
+  __shared__ double sXj[DT * NB];
   for (int q = tid; q < d * NB; q += TPB) {
     const int k = q / NB, r = q % NB;
     sXi[q] = (i0 + r < n) ? S.X[(size_t)k * n + i0 + r] : 0.0;
@@ -1285,8 +1285,8 @@ template <int DT>
 __global__ __launch_bounds__(TPB) void k_cross_cov(PredictArgs a) {
   const int j0 = blockIdx.x * NB, i0 = blockIdx.y * NB, tid = threadIdx.x;
   const int n = a.n, m = a.m, d = a.d;
-  __shared__ double sXo[DMAX * NB];
-  __shared__ double sXn[DMAX * NB];
+  __shared__ double sXo[DT * NB];
+  __shared__ double sXn[DT * NB];
   for (int q = tid; q < d * NB; q += TPB) {
     const int k = q / NB, r = q % NB;
     sXo[q] = (i0 + r < n) ? a.X[(size_t)k * n + i0 + r] : 0.0;
@@ -1322,8 +1322,8 @@ template <int DT>
 __global__ __launch_bounds__(TPB) void k_cov_mat(CovMatArgs a) {
   const int j0 = blockIdx.x * NB, i0 = blockIdx.y * NB, tid = threadIdx.x;
   const int n1 = a.n1, n2 = a.n2, d = a.d;
-  __shared__ double s1[DMAX * NB];
-  __shared__ double s2[DMAX * NB];
+  __shared__ double s1[DT * NB];
+  __shared__ double s2[DT * NB];
   for (int q = tid; q < d * NB; q += TPB) {
     const int k = q / NB, r = q % NB;
     s1[q] = (i0 + r < n1) ? a.X1[(size_t)k * n1 + i0 + r] : 0.0;
@@ -1560,8 +1560,8 @@ template <int DT>
 __global__ __launch_bounds__(TPB) void k_predict_dron(PredictArgs a) {
   const int j0 = blockIdx.x * NB, i0 = blockIdx.y * NB, tid = threadIdx.x;
   const int n = a.n, m = a.m, d = a.d;
-  __shared__ double sXo[DMAX * NB];
-  __shared__ double sXn[DMAX * NB];
+  __shared__ double sXo[DT * NB];
+  __shared__ double sXn[DT * NB];
   for (int q = tid; q < d * NB; q += TPB) {
     const int k = q / NB, r = q % NB;
     sXo[q] = (i0 + r < n) ? a.X[(size_t)k * n + i0 + r] : 0.0;

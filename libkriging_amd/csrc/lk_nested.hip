@@ -37,8 +37,8 @@ __global__ __launch_bounds__(TPB) void k_nk_cross(NkArgs a) {
   const NkGroup G = a.groups[a.tile_group[blockIdx.y]];
   const int d = a.d, qc = a.qc, ld = a.Npad;
   const int nvalid = G.off + G.n;  // rows >= nvalid are the group's padding
-  __shared__ double sXo[DMAX * NB];
-  __shared__ double sXn[DMAX * NB];
+  __shared__ double sXo[DT * NB];
+  __shared__ double sXn[DT * NB];
   for (int q = tid; q < d * NB; q += TPB) {
     const int k = q / NB, r = q % NB;
     sXo[q] = a.Xg[(size_t)k * ld + i0 + r];
