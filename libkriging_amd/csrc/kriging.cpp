@@ -181,9 +181,22 @@ ModelFlags Kriging::flags_for_eval() const {
 }
 
 // Optim::theta_bounds (Optim.cpp:179-209)
-void Kriging::theta_bounds(std::vector<double>& lo, std::vector<double>& up) const {
+void Kriging::theta_bounds(std::vector<double>& lo, std::vector<double>& up) {
+  if (!pre_lo_.empty() && (int)pre_lo_.size() == d_ && (int)pre_up_.size() == d_ && !normalize_) {
+    lo = pre_lo_;
+    up = pre_up_;
+    pre_lo_.clear();
+    pre_up_.clear();
+    return;
+  }
+  pre_lo_.clear();
+  pre_up_.clear();
+  theta_bounds_of(X_.data(), y_.data(), n_, d_, lo, up);
+}
+
+void Kriging::theta_bounds_of(const double* X_, const double* y_, int n, int d, std::vector<double>& lo,
+                              std::vector<double>& up) {
   const OptimConfig& oc = optim_config();
-  const int n = n_, d = d_;
   lo.assign(d, 0.0);
   up.assign(d, 0.0);
   for (int k = 0; k < d; ++k) {

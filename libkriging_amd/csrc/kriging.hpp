@@ -177,8 +177,25 @@ class Kriging {
   EvalRequest commit_request(const std::vector<double>& gamma) const;
   void commit_apply(const std::vector<double>& theta, double alpha, int slot, const EvalResult& r);
   void commit(const std::vector<double>& theta, double alpha);
-  void theta_bounds(std::vector<double>& lo, std::vector<double>& up) const;
+  void theta_bounds(std::vector<double>& lo, std::vector<double>& up);
 
+ public:
+  // Optim::theta_bounds (Optim.cpp:179-209) of a design X (n x d,
+  // column-major) and response y: [lower_factor, upper_factor] x the column
+  // ranges, the lower bound raised by the variogram heuristic (an O(n^2 d)
+  // pair loop, the order of the reference's)
+  static void theta_bounds_of(const double* X, const double* y, int n, int d, std::vector<double>& lo,
+                              std::vector<double>& up);
+  // bounds the next begin_fit uses instead of computing them (a caller that
+  // computed them for many models at once, in parallel); they must be those
+  // of the data that begin_fit receives, without normalisation
+  void use_theta_bounds(std::vector<double> lo, std::vector<double> up) {
+    pre_lo_ = std::move(lo);
+    pre_up_ = std::move(up);
+  }
+
+ private:
+  std::vector<double> pre_lo_, pre_up_;
   std::string kernel_;
   NoiseModel noise_;
   int device_;

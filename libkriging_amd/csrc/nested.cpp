@@ -1,7 +1,9 @@
 #include "nested.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <thread>
 #include <limits>
 #include <random>
 #include <sstream>
@@ -274,16 +276,39 @@ void NestedKriging::fit_impl(RegModel regmodel, const std::string& optim, const 
   std::vector<std::unique_ptr<FitJob>> jobs;
   std::vector<FitJob*> jp;
   std::vector<double> Xg, yg;
-  for (int g = 0; g < p; ++g) {
+  auto gather = [&](int g, std::vector<double>& Xo, std::vector<double>& yo) {
     const std::vector<int>& idx = groups_[g];
     const int ng = (int)idx.size();
-    Xg.resize((size_t)ng * d);
-    yg.resize(ng);
+    Xo.resize((size_t)ng * d);
+    yo.resize(ng);
     for (int a = 0; a < ng; ++a) {
-      yg[a] = y_[idx[a]];
-      for (int k = 0; k < d; ++k) Xg[a + (size_t)k * ng] = X_[idx[a] + (size_t)k * n];
+      yo[a] = y_[idx[a]];
+      for (int k = 0; k < d; ++k) Xo[a + (size_t)k * ng] = X_[idx[a] + (size_t)k * n];
     }
+  };
+  // the groups' theta bounds (each an O(n_g^2 d) host pair loop, in the
+  // reference's order) computed for all groups at once on the host cores
+  std::vector<std::vector<double>> blo(p), bup(p);
+  if (optim != "none") {
+    const int nt = std::max(1, std::min<int>(p, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::atomic<int> next{0};
+    auto work = [&] {
+      std::vector<double> Xl, yl;
+      for (int g = next++; g < p; g = next++) {
+        gather(g, Xl, yl);
+        Kriging::theta_bounds_of(Xl.data(), yl.data(), (int)groups_[g].size(), d, blo[g], bup[g]);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+  }
+  for (int g = 0; g < p; ++g) {
+    const int ng = (int)groups_[g].size();
+    gather(g, Xg, yg);
     sub_.push_back(std::make_unique<Kriging>(kernel_));
+    if (optim != "none") sub_.back()->use_theta_bounds(blo[g], bup[g]);
     jobs.push_back(sub_.back()->begin_fit(yg.data(), Xg.data(), ng, d, regmodel, false, optim, objective, parameters));
     jp.push_back(jobs.back().get());
   }
