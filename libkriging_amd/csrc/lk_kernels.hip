@@ -911,10 +911,13 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
 // same contraction with B = R^-1 - R^-1 F (F'R^-1 F)^-1 F'R^-1 = R^-1 - G'G in
 // place of R^-1 (G = R*^-T F*' L^-1, columns of V from the leave-one-out
 // pass k_loo_g), subtracted from each tile before the contraction.
-// grid (nblk_max (nblk_max + 1) / 2, nslots), 256 threads
+// grid (nblk_max (nblk_max + 1) / 2, nslots), 256 threads.  Waves per SIMD:
+// 4 up to 8 dimensions (128 registers, a few values spilled outside the
+// MFMA loop: 0.717 -> 0.680 ms per 64 x n = 1024 launch against 3 waves), 2
+// for 9..16 dimensions (1 before: 9.5 -> 8 ms at n = 8192, d = 10)
 // ---------------------------------------------------------------------------
 template <int DT, bool LMP>
-__global__ __launch_bounds__(TPB, (DT == 16 ? 2 : 1)) void k_lauum_adjoint(const Slot* __restrict__ slots,
+__global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lauum_adjoint(const Slot* __restrict__ slots,
                                                        const SlotParams* __restrict__ params,
                                                        SlotOut* __restrict__ out) {
   const TileSlot ts = tile_slot();
