@@ -3,8 +3,8 @@
 For each size bucket of the batched pipeline (largest matrix of at most 4,
 8, 16, 32, 64, 128 64-row blocks, and the rest) it times batched LL +
 gradient evaluations of a representative workload under every candidate
-(Cholesky panel width P in block columns, stream parts S) and keeps the
-fastest.  The table is written as JSON; a context loads it at creation from
+(Cholesky panel width P in block columns, stream parts S, trailing updates
+on 64 x 64 or 128 x 64 tiles W) and keeps the fastest.  The table is written as JSON; a context loads it at creation from
 tuning_<arch>.json beside liblkgpu.so (or the file LKGPU_TUNING names).
 
 Run on the GPU the table is for:
@@ -29,6 +29,7 @@ from . import _native as N
 BUCKETS = [4, 8, 16, 32, 64, 128, None]
 PANELS = [1, 2, 4, 8]
 STREAMS = [1, 2, 4]
+WIDE = [0, 1]
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tuning_gfx950.json")
 
@@ -37,12 +38,12 @@ def _batch_size(n):
     return 64 if n <= 1024 else 16 if n <= 2048 else 4 if n <= 4096 else 1
 
 
-def _table(panel, streams):
-    return {"buckets": [{"max_nblk": b, "panel_blocks": panel, "streams": streams} for b in BUCKETS]}
+def _table(panel, streams, wide):
+    return {"buckets": [{"max_nblk": b, "panel_blocks": panel, "streams": streams, "wide": wide} for b in BUCKETS]}
 
 
-def time_config(ctx, pid, theta, panel, streams, reps):
-    ctx.set_tuning(_table(panel, streams))
+def time_config(ctx, pid, theta, panel, streams, wide, reps):
+    ctx.set_tuning(_table(panel, streams, wide))
     ids = [pid] * theta.shape[0]
     ctx.ll_grad_batch(ids, theta, True)  # capture the graph of this layout
     ctx.ll_grad_batch(ids, theta, True)
@@ -69,7 +70,7 @@ def tune(reps=5, quick=False, log=print):
             continue
         n = 64 * b
         if quick and n > 2048:
-            out.append({"max_nblk": b, "panel_blocks": 4, "streams": 1 if _batch_size(n) < 8 else 4,
+            out.append({"max_nblk": b, "panel_blocks": 4, "streams": 1 if _batch_size(n) < 8 else 4, "wide": 0,
                         "ms": None, "note": "not measured (--quick)"})
             continue
         B = _batch_size(n)
@@ -85,14 +86,15 @@ def tune(reps=5, quick=False, log=print):
             for S in STREAMS:
                 if B % (8 * S) != 0 and S > 1:
                     continue  # this batch would not split into S parts
-                ms = time_config(ctx, pid, theta, P, S, reps)
-                results[f"P{P}_S{S}"] = round(ms, 4)
-                if best is None or ms < best[0]:
-                    best = (ms, P, S)
+                for W in WIDE:
+                    ms = time_config(ctx, pid, theta, P, S, W, reps)
+                    results[f"P{P}_S{S}_W{W}"] = round(ms, 4)
+                    if best is None or ms < best[0]:
+                        best = (ms, P, S, W)
         ctx.drop_problem(pid)
-        log(f"bucket <= {b} blocks (n = {n}, batch {B}): best P = {best[1]}, S = {best[2]}, {best[0]:.3f} ms; "
-            f"all {results}")
-        out.append({"max_nblk": b, "panel_blocks": best[1], "streams": best[2], "ms": round(best[0], 4),
+        log(f"bucket <= {b} blocks (n = {n}, batch {B}): best P = {best[1]}, S = {best[2]}, W = {best[3]}, "
+            f"{best[0]:.3f} ms; all {results}")
+        out.append({"max_nblk": b, "panel_blocks": best[1], "streams": best[2], "wide": best[3], "ms": round(best[0], 4),
                     "n": n, "batch": B, "candidates_ms": results})
     ctx.close()
     return {"arch": "gfx950", "workload": "LL+grad, matern5_2, d = 6, constant trend", "buckets": out}

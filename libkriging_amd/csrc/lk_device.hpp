@@ -276,6 +276,120 @@ __device__ inline void tile_mainloop(const double* __restrict__ A, int lda, int 
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// Wide tiles: 128 rows (two 64-row blocks of A) x 64 columns per 256-thread
+// workgroup, both operands plain (i contiguous in memory); wave (wr, wc) owns
+// rows 64 wr .. 64 wr + 63 and columns 32 wc .. 32 wc + 31 as 4 x 2 MFMA
+// blocks.  Per K-chunk a wave issues 8 MFMAs for 6 LDS operand reads (4 for
+// 4 in the 64 x 64 tile) and the workgroup stages 3 NB x KC operand slices
+// for twice the flops: half the barriers and 3/4 of the staging per flop.
+// ---------------------------------------------------------------------------
+constexpr int WROWS = 2 * NB;
+constexpr int LDSW = WROWS + 2;  // row stride of the 128-row staging (16-byte aligned pairs)
+
+template <int S>
+__device__ inline void chunk_store_plain(const ChunkRegs& r, double* __restrict__ dst, int tid) {
+  const int ii = 2 * (tid & 31);
+#pragma unroll
+  for (int q = 0; q < CHUNK_V2; ++q) {
+    const int kk = (tid >> 5) + 8 * q;
+    dst[kk * S + ii] = r.v[q].x;
+    dst[kk * S + ii + 1] = r.v[q].y;
+  }
+}
+
+__device__ inline void acc_zero_w(d4 (&acc)[4][2]) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+}
+__device__ inline int accw_row(int lane, int wr, int bi) { return wr * 64 + bi * 16 + (lane & 15); }
+__device__ inline int accw_col(int lane, int wc, int bj, int e) { return wc * 32 + bj * 16 + (lane >> 4) + 4 * e; }
+
+// rows >= rows_valid are neither loaded nor stored (a half tile)
+__device__ inline void accw_load(d4 (&acc)[4][2], const double* __restrict__ C, int ld, int lane, int wr, int wc,
+                                 int rows_valid) {
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = accw_row(lane, wr, bi);
+        acc[bi][bj][e] = row < rows_valid ? gload(C + (size_t)accw_col(lane, wc, bj, e) * ld + row) : 0.0;
+      }
+}
+__device__ inline void accw_store(const d4 (&acc)[4][2], double* __restrict__ C, int ld, int lane, int wr, int wc,
+                                  int rows_valid) {
+#pragma unroll
+  for (int bi = 0; bi < 4; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = accw_row(lane, wr, bi);
+        if (row < rows_valid) C[(size_t)accw_col(lane, wc, bj, e) * ld + row] = acc[bi][bj][e];
+      }
+}
+
+// acc (+)= A(ai0 + 0..127, k) B(bj0 + 0..63, k)^T over k in [kbeg, kend)
+// (a multiple of KC), both operands plain (A[k * lda + i]).  When
+// rows_valid <= 64 the second row block is not read (its rows are staged as
+// a copy of the first and their results are discarded by the caller).
+// sbuf: 2 (KC LDSW + KC LDSS) doubles.
+template <bool NEG>
+__device__ inline void tile_mainloop_w(const double* __restrict__ A, int lda, int ai0, const double* __restrict__ B,
+                                       int ldb, int bj0, int kbeg, int kend, double* __restrict__ sbuf,
+                                       d4 (&acc)[4][2], int tid, int lane, int wr, int wc, int rows_valid) {
+  if (kbeg >= kend) return;
+  double* sA[2] = {sbuf, sbuf + KC * (LDSW + LDSS)};
+  double* sB[2] = {sbuf + KC * LDSW, sbuf + KC * (2 * LDSW + LDSS)};
+  const int ai1 = rows_valid > NB ? ai0 + NB : ai0;
+  constexpr int D = 2;  // chunks in flight (3 x 8 registers each)
+  ChunkRegs ra0[D], ra1[D], rb[D];
+  const int nch = (kend - kbeg) / KC;
+#pragma unroll
+  for (int q = 0; q < D; ++q)
+    if (q < nch) {
+      chunk_load<false>(ra0[q], A, lda, ai0, kbeg + q * KC, tid);
+      chunk_load<false>(ra1[q], A, lda, ai1, kbeg + q * KC, tid);
+      chunk_load<false>(rb[q], B, ldb, bj0, kbeg + q * KC, tid);
+    }
+  const int r = lane & 15, kq = lane >> 4;
+  for (int t0 = 0; t0 < nch; t0 += 2) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int t = t0 + u;
+      if (t >= nch) break;
+      chunk_store_plain<LDSW>(ra0[u], sA[u], tid);
+      chunk_store_plain<LDSW>(ra1[u], sA[u] + NB, tid);
+      chunk_store_plain<LDSS>(rb[u], sB[u], tid);
+      __syncthreads();
+      if (t + D < nch) {
+        chunk_load<false>(ra0[u], A, lda, ai0, kbeg + (t + D) * KC, tid);
+        chunk_load<false>(ra1[u], A, lda, ai1, kbeg + (t + D) * KC, tid);
+        chunk_load<false>(rb[u], B, ldb, bj0, kbeg + (t + D) * KC, tid);
+      }
+#pragma unroll
+      for (int k0 = 0; k0 < KC; k0 += 4) {
+        const double* pa = sA[u] + (k0 + kq) * LDSW + wr * 64 + r;
+        const double* pb = sB[u] + (k0 + kq) * LDSS + wc * 32 + r;
+        double av[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) av[i] = NEG ? -pa[16 * i] : pa[16 * i];
+        const double b0 = pb[0], b1 = pb[16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          acc[i][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, av[i], acc[i][0], 0, 0, 0);
+          acc[i][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, av[i], acc[i][1], 0, 0, 0);
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
 // store the transpose: element (row, col) of the tile goes to C[row * ld + col]
 __device__ inline void acc_store_t(const d4 (&acc)[2][2], double* __restrict__ C, int ld, int lane, int wr,
                                    int wc) {

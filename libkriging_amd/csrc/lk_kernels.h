@@ -139,6 +139,8 @@ hipError_t launch_potrf_panel(const Batch& b, int k);
 // A(r, c) -= sum_{k in [kb, ke)} L(r, k) L(c, k)^T, tile columns c in [c0, c1]
 // (c1 < 0: to the last column), rows r >= c
 hipError_t launch_potrf_update(const Batch& b, int kb, int ke, int c0, int c1);
+// the trailing update (every column from c0 on) on 128 x 64 tiles
+hipError_t launch_potrf_update_trailing(const Batch& b, int kb, int ke, int c0);
 // the same update of a one-slot batch by `workgroups` persistent workgroups
 hipError_t launch_potrf_update_persistent(const Batch& b, int kb, int ke, int c0, int c1, int workgroups);
 hipError_t launch_trtri_level(const Batch& b, int s, int step);

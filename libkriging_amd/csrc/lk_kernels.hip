@@ -583,6 +583,44 @@ __global__ __launch_bounds__(TPB) void k_potrf_update(const Slot* __restrict__ s
   update_tile(S, r, c, kb, ke, sbuf);
 }
 
+// The trailing update (every column from c0 on) on 128 x 64 tiles: column c's
+// row blocks c, c+1, ... taken in pairs (the last one alone when their count
+// is odd).  Tile t -> (c, first row block).  grid (tiles, nslots)
+__device__ inline bool update_tile_w(int tile, int nb, int c0, int& r, int& c) {
+  int t = tile;
+  for (c = c0; c < nb; ++c) {
+    const int pairs = (nb - c + 1) / 2;
+    if (t < pairs) {
+      r = c + 2 * t;
+      return true;
+    }
+    t -= pairs;
+  }
+  return false;
+}
+
+__global__ __launch_bounds__(TPB, 2) void k_potrf_update_w(const Slot* __restrict__ slots,
+                                                        const SlotParams* __restrict__ params,
+                                                        SlotOut* __restrict__ out, int kb, int ke, int c0) {
+  const TileSlot ts = tile_slot();
+  const int b = ts.slot;
+  const Slot S = slots[b];
+  if (!slot_live(params[b], out[b]) || params[b].mode == MODE_FACTOR) return;
+  const int nb = S.nblk;
+  if (c0 >= nb || ke > nb) return;
+  int r, c;
+  if (!update_tile_w(ts.tile, nb, c0, r, c)) return;
+  const int i0 = r * NB, j0 = c * NB, ld = S.npad;
+  const int rows_valid = (r + 1 < nb) ? WROWS : NB;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wr = w >> 1, wc = w & 1;
+  __shared__ double sbuf[2 * KC * (LDSW + LDSS)];
+  d4 acc[4][2];
+  double* C = S.A + (size_t)j0 * ld + i0;
+  accw_load(acc, C, ld, lane, wr, wc, rows_valid);
+  tile_mainloop_w<true>(S.A, ld, i0, S.A, ld, j0, kb * NB, ke * NB, sbuf, acc, tid, lane, wr, wc, rows_valid);
+  accw_store(acc, C, ld, lane, wr, wc, rows_valid);
+}
+
 // The same update for one slot by a persistent grid: workgroup g takes tiles
 // g, g + G, g + 2G, ... (G = gridDim.x).  Launched with about two
 // workgroups per CU, it leaves room on every CU (LDS and registers) for the
@@ -1737,6 +1775,15 @@ hipError_t launch_potrf_update(const Batch& B, int kb, int ke, int c0, int c1) {
   if (tiles <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_potrf_update, dim3(tiles, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, kb, ke,
                      c0, c1);
+  return hipGetLastError();
+}
+hipError_t launch_potrf_update_trailing(const Batch& B, int kb, int ke, int c0) {
+  const int nb = B.nblk_max;
+  if (c0 >= nb) return hipSuccess;
+  int tiles = 0;
+  for (int c = c0; c < nb; ++c) tiles += (nb - c + 1) / 2;
+  hipLaunchKernelGGL(k_potrf_update_w, dim3(tiles, B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, kb, ke,
+                     c0);
   return hipGetLastError();
 }
 hipError_t launch_potrf_update_persistent(const Batch& B, int kb, int ke, int c0, int c1, int workgroups) {

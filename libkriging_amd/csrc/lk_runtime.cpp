@@ -107,6 +107,7 @@ void Runtime::load_tuning_json(const std::string& text) {
     q.max_nblk = mx >= (double)INT_MAX ? INT_MAX : (int)mx;
     q.panel_blocks = (int)b.at("panel_blocks").as_number();
     q.streams = (int)b.at("streams").as_number();
+    if (const io::Json* w = b.find("wide")) q.wide = w->as_number() != 0.0 ? 1 : 0;
     t.push_back(q);
   }
   set_tuning(t);
@@ -535,7 +536,7 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
     parts[0].dmax = std::max(parts[0].dmax, parts[1].dmax);
     parts.erase(parts.begin() + 1);
   }
-  for (Part& q : parts) q.panel_blocks = tuning_[bucket_of(q.nblk_max)].panel_blocks;
+  for (Part& q : parts) apply_tuning(q);
   buckets_ = parts;
   slot_of_req_ = slot_of_req;
   slot_problem_ = mapping;
@@ -548,8 +549,14 @@ void Runtime::ensure_slots(const std::vector<EvalRequest>& reqs) {
 // The pipeline as an ordered list of launches, each tagged with the stages
 // it belongs to (bit s = stage s): 0 build, 1 Cholesky, 2 first trailing
 // update, 3 trtri, 4 triangular solves + LL, 5 fused R^-1 / adjoint.
+void Runtime::apply_tuning(Part& q) const {
+  const BucketTuning& t = tuning_[bucket_of(q.nblk_max)];
+  q.panel_blocks = t.panel_blocks;
+  q.wide = t.wide;
+}
+
 std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_grad, int objective,
-                                                   int panel_blocks, bool lookahead) const {
+                                                   int panel_blocks, int wide, bool lookahead) const {
   std::vector<Step> v;
   v.push_back({1u << 0, "build", [B] { return launch_build(B); }});
   // blocked right-looking Cholesky, two levels: panels of P block columns;
@@ -612,7 +619,9 @@ std::vector<Runtime::Step> Runtime::pipeline_steps(const Batch& B, bool want_gra
     }
     if (K1 < B.nblk_max) {
       v.push_back({(1u << 1) | (first_update ? 1u << 2 : 0u), "potrf_update (trailing)",
-                   [B, K0, K1] { return launch_potrf_update(B, K0, K1, K1, -1); }});
+                   [B, K0, K1, wide] {
+                     return wide ? launch_potrf_update_trailing(B, K0, K1, K1) : launch_potrf_update(B, K0, K1, K1, -1);
+                   }});
       first_update = false;
     }
   }
@@ -712,7 +721,8 @@ void Runtime::take_stage_time(double* avg_ms, int* count) {
 // bucket's tuned streams) equal parts whose slot counts are multiples of 8
 // (each part keeps the XCD-aware placement).
 std::vector<Runtime::Part> Runtime::plan_parts(bool timed, BenchMap* bench) const {
-  Part all{0, nslots_, nblk_max_, dmax_, tuning_[bucket_of(nblk_max_)].panel_blocks};
+  Part all{0, nslots_, nblk_max_, dmax_};
+  apply_tuning(all);
   if (timed || bench != nullptr || noalloc_) return {all};
   if (buckets_.size() > 1 && streams_ > 1) {
     std::vector<Part> v = buckets_;
@@ -720,7 +730,7 @@ std::vector<Runtime::Part> Runtime::plan_parts(bool timed, BenchMap* bench) cons
       v[0].count += v[1].count;
       v[0].nblk_max = std::max(v[0].nblk_max, v[1].nblk_max);
       v[0].dmax = std::max(v[0].dmax, v[1].dmax);
-      v[0].panel_blocks = tuning_[bucket_of(v[0].nblk_max)].panel_blocks;
+      apply_tuning(v[0]);
       v.erase(v.begin() + 1);
     }
     return v;
@@ -770,9 +780,10 @@ void Runtime::issue_split(const std::vector<Step>* parts, int S) {
 void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
   hip_check(hipMemcpyAsync(d_params_, h_params_, sizeof(SlotParams) * nslots_, hipMemcpyHostToDevice, stream_),
             "upload params");
-  const Part all{0, nslots_, nblk_max_, dmax_, tuning_[bucket_of(nblk_max_)].panel_blocks};
-  const std::vector<Step> steps =
-      pipeline_steps(part_batch(all, stream_), want_grad, objective, all.panel_blocks, lookahead_ && nslots_ == 1);
+  Part all{0, nslots_, nblk_max_, dmax_};
+  apply_tuning(all);
+  const std::vector<Step> steps = pipeline_steps(part_batch(all, stream_), want_grad, objective, all.panel_blocks,
+                                                 all.wide, lookahead_ && nslots_ == 1);
   const auto rng = stage_range(steps, timed_stage_);
   const bool timed = rng.first < rng.second;
   const std::vector<Part> plan = plan_parts(timed, bench);
@@ -783,7 +794,7 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
   if (split) {
     for (int q = 0; q < S; ++q)
       part[q] = pipeline_steps(part_batch(plan[q], q == 0 ? stream_ : xstream_[q - 1]), want_grad, objective,
-                               plan[q].panel_blocks);
+                               plan[q].panel_blocks, plan[q].wide);
   }
   if (bench != nullptr) {
     // uncaptured run with one event pair around the whole pipeline

@@ -94,6 +94,7 @@ struct BucketTuning {
   int max_nblk = 0;       // upper bound of the bucket, in 64-row blocks
   int panel_blocks = 4;   // block columns per Cholesky panel
   int streams = 4;        // parts (1, 2, 4) a batch of this bucket alone is split into
+  int wide = 0;           // trailing Cholesky updates on 128 x 64 tiles (else 64 x 64)
 };
 
 // Per-evaluation timings filled when a bench map is requested (Bench::toc
@@ -218,12 +219,13 @@ class Runtime {
   };
 
  private:
-  std::vector<Step> pipeline_steps(const Batch& B, bool want_grad, int objective, int panel_blocks,
+  std::vector<Step> pipeline_steps(const Batch& B, bool want_grad, int objective, int panel_blocks, int wide,
                                    bool lookahead = false) const;
   // a contiguous slot range run as one unit of launches
   struct Part {
-    int first = 0, count = 0, nblk_max = 0, dmax = 0, panel_blocks = 4;
+    int first = 0, count = 0, nblk_max = 0, dmax = 0, panel_blocks = 4, wide = 0;
   };
+  void apply_tuning(Part& q) const;
   std::vector<Part> plan_parts(bool timed, BenchMap* bench) const;
   Batch part_batch(const Part& q, hipStream_t s) const;
   hipGraph_t capture(const std::vector<Step>& v, size_t a, size_t b);
