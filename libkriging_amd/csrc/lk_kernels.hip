@@ -126,18 +126,21 @@ __global__ __launch_bounds__(TPB) void k_build(const Slot* __restrict__ slots, c
 }
 
 // ---------------------------------------------------------------------------
-// (2a) Cholesky of the diagonal block k and its inverse, one 256-thread
-// workgroup per slot.  Lane l is row l (factor) or column l (inverse); wave w
-// owns columns (factor) or rows (inverse) 16w .. 16w+15 in registers.  Both
-// sweeps go 4 columns / rows at a time: a 4 x 4 diagonal sub-block is
-// factored redundantly by every lane from LDS broadcasts, then a rank-4
-// update; 16 sequential steps per sweep instead of 64, one workgroup barrier
-// per step (every wave computes the L values of all 64 rows and reads its
-// columns' values back from its own LDS copy).  Pivot square roots come from
-// v_rsq_f64 refined by one third-order step.  Writes L_kk to A and
-// L_kk^-1 to the same block of W (both with zero strict upper part), the
-// partial log-det, and flags a non-positive / NaN pivot (dpotrf failure,
-// LinearAlgebra.cpp:71).
+// (2a) Cholesky of the diagonal block k and its inverse, one 512-thread
+// workgroup per slot: waves 0-3 run the factor sweep, waves 4-7 the inverse
+// sweep.  Both go 4 columns / rows at a time (16 sequential steps instead of
+// 64): a 4 x 4 diagonal sub-block is factored redundantly by every factor
+// lane (lane = row) from LDS broadcasts, each lane solves its row's 4 L
+// values, and the rank-4 update of the later columns runs on MFMA
+// (V_MFMA_F64_16X16X4_F64, one 16 x 16 tile per instruction): its operands
+// are one value per lane from LDS instead of a broadcast read per updated
+// element, which kept the LDS return path the bound of the step.  Factor
+// wave w holds columns 16w .. 16w+15 of the block as four 16 x 16 MFMA
+// tiles; inverse wave w holds rows 16w .. 16w+15 of X = L_kk^-1 the same
+// way.  Pivot square roots come from v_rsq_f64 refined by one third-order
+// step.  Writes L_kk to A and L_kk^-1 to the same block of W (both with
+// zero strict upper part), the partial log-det, and flags a non-positive /
+// NaN pivot (dpotrf failure, LinearAlgebra.cpp:71).
 // ---------------------------------------------------------------------------
 __device__ inline void pivot_sqrt(double a, double& root, double& rroot) {
   // y = v_rsq_f64(a) (relative error below 2^-22), then one third-order
@@ -154,44 +157,52 @@ __device__ inline void pivot_sqrt(double a, double& root, double& rroot) {
   rroot = iv;
 }
 
-// columns (factor) / rows (inverse) per step of the diagonal-block sweeps
-#ifndef LK_DIAG_CW
-#define LK_DIAG_CW 4
-#endif
-constexpr int DCW = LK_DIAG_CW;
-static_assert(DCW == 4 || DCW == 8, "diagonal sweep step must be 4 or 8 columns");
+// columns (factor) / rows (inverse) per step of the diagonal-block sweeps:
+// the K of one MFMA
+constexpr int DCW = 4;
 
 struct DiagLds {
   // LDS layouts keep a wave's 64 lanes on consecutive doubles for every
   // per-lane access (stride 1 writes / reads, or one broadcast address)
   double P[2][DCW * NB];  // current values of the step's columns, [column][row]
-  double Q[4][DCW * NB];  // their L values, [column][row], one copy per wave
+  double Q[4][DCW * NB];  // their L values, [column][row], one copy per factor wave
   double L[NB * (NB + 2)];  // L_kk column-major, stride NB + 2 (16-byte aligned column pairs)
   double X[NB * (NB + 1)];  // L_kk^-1 rows staged for coalesced stores, stride NB + 1
-  double R[2][DCW * NB];    // the step's rows of L_kk^-1 in the inverse sweep
+  double R[2][DCW * NB];    // the step's rows of L_kk^-1 in the inverse sweep, [row][column]
+  double Z[DCW * NB];       // the inverse owner's accumulated step rows, [row][column]
   double diag[NB], rinv[NB];
 };
 
-// One step of the factor sweep for wave W (owns columns 16W .. 16W+15),
-// split at its workgroup barrier.  Step p factors columns j0 = DCW p ..
-// j0 + DCW - 1.  Before the barrier the owner of those columns publishes
-// their current values.
+// MFMA tile layouts (lane l, register e; the instruction returns D(a, b) with
+// a = (l >> 4) + 4e, b = l & 15 for D = P Q, P (16 x 4) given as lane l ->
+// P(l & 15, l >> 4) and Q (4 x 16) as lane l -> Q(l >> 4, l & 15)):
+//   factor tile rb of wave W: element (16 rb + (l & 15), 16 W + (l >> 4) + 4e)
+//     (D transposed: the update's P is the wave's columns, Q the tile's rows)
+//   inverse tile cb of wave W: element (16 W + (l >> 4) + 4e, 16 cb + (l & 15))
+
+// Before the barrier of step p: the owner of the step's columns publishes
+// their current values (tile register e = (j0 & 15) / 4 holds column
+// j0 + (l >> 4) of the tile's 16 rows).  Rows above 16 W are not written:
+// no lane reads them (their L values are forced to zero).
 template <int W>
-__device__ inline void factor_publish(int p, const double (&a)[16], DiagLds& sh, int l) {
+__device__ inline void factor_publish(int p, const d4 (&T)[4], DiagLds& sh, int l) {
   const int j0 = DCW * p;
   if (W == j0 / 16) {
+    double* P = sh.P[p & 1];
+    const int e = (j0 & 15) / 4;
 #pragma unroll
-    for (int s = 0; s < DCW; ++s) sh.P[p & 1][s * NB + l] = a[(j0 & 15) + s];
+    for (int rb = W; rb < 4; ++rb) P[(l >> 4) * NB + 16 * rb + (l & 15)] = T[rb][e];
   }
 }
 
 // After the barrier: the DCW x DCW diagonal sub-block (factored by every lane
-// from LDS broadcasts), this row's L values in the step's columns, the
-// rank-DCW update of the wave's later columns; the owner also writes the
-// step's columns of L (zero above the diagonal) to sh.L, and wave 0 their
-// diagonal and reciprocal, which the inverse sweep reads two steps later.
+// from LDS broadcasts), this row's L values in the step's columns, then the
+// rank-DCW update of the wave's columns right of the step on MFMA; the owner
+// also writes the step's columns of L (zero above the diagonal) to sh.L, and
+// wave 0 their diagonal and reciprocal, which the inverse sweep reads two
+// steps later.
 template <int W>
-__device__ inline bool factor_step(int p, double (&a)[16], DiagLds& sh, int l) {
+__device__ inline bool factor_step(int p, d4 (&T)[4], DiagLds& sh, int l) {
   bool ok = true;
   const int j0 = DCW * p;
   const double* P = sh.P[p & 1];
@@ -232,26 +243,24 @@ __device__ inline bool factor_step(int p, double (&a)[16], DiagLds& sh, int l) {
     }
     // rows inside the block reproduce L4 below the diagonal (same
     // operations); the diagonal takes the corrected root, rows above the
-    // block are zero.  Entries above the diagonal are left as computed:
-    // they only reach the strict upper part of this tile's columns, which
-    // no lower-part value reads and the stores zero.
+    // step are zero (their published values are not meaningful).  Entries
+    // above the diagonal inside the block are left as computed: they only
+    // reach the strict upper part of the tiles' columns, which no
+    // lower-part value reads and the stores zero.
 #pragma unroll
     for (int s = 0; s < DCW; ++s) {
       x[s] = (l == j0 + s) ? L4[s][s] : x[s];
       x[s] = (l < j0) ? 0.0 : x[s];
     }
   }
-  // every wave computed the same x for all 64 rows: each keeps its own copy
-  // and reads its columns' values back without a workgroup barrier (one
-  // wave's LDS operations complete in order; the fences keep the compiler
-  // from moving the reads above the writes)
   if (W == j0 / 16) {
 #pragma unroll
-    for (int s = 0; s < DCW; ++s) {
-      a[(j0 & 15) + s] = x[s];
-      sh.L[(j0 + s) * (NB + 2) + l] = (l >= j0 + s) ? x[s] : 0.0;
-    }
+    for (int s = 0; s < DCW; ++s) sh.L[(j0 + s) * (NB + 2) + l] = (l >= j0 + s) ? x[s] : 0.0;
   }
+  // every wave computed the same x for all 64 rows: each keeps its own copy
+  // and reads the MFMA operands back without a workgroup barrier (one
+  // wave's LDS operations complete in order; the fences keep the compiler
+  // from moving the reads above the writes)
 #pragma unroll
   for (int s = 0; s < DCW; ++s) Q[s * NB + l] = x[s];
   if (W == 0 && l == 0) {
@@ -264,38 +273,45 @@ __device__ inline bool factor_step(int p, double (&a)[16], DiagLds& sh, int l) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // rank-DCW update of this wave's columns right of the block
+  // tile rb -= X_rb X_W^T over the wave's columns right of the step (the
+  // P operand is zero for the columns up to j0 + DCW - 1, which stay)
+  if (W >= j0 / 16 && 16 * W + 15 >= j0 + DCW) {
+    const int c = 16 * W + (l & 15);
+    const double pv = (c >= j0 + DCW) ? -Q[(l >> 4) * NB + c] : 0.0;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int c = 16 * W + q;
-    if (c >= j0 + DCW) {
-      double t = a[q];
-#pragma unroll
-      for (int s = 0; s < DCW; ++s) t = fma(-x[s], Q[s * NB + c], t);
-      a[q] = t;
+    for (int rb = W; rb < 4; ++rb) {
+      const double qv = Q[(l >> 4) * NB + 16 * rb + (l & 15)];
+      T[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv, qv, T[rb], 0, 0, 0);
     }
   }
   return ok;
 }
 
 // One step of the inverse sweep for wave W (owns rows 16W .. 16W+15 of
-// X = L_kk^-1; lane = column), split at its barrier.  Step p solves rows
-// j0 = DCW p .. j0 + DCW - 1 (owner) after reading the wave's L(i, j0 ..)
-// for the update that follows the barrier.
+// X = L_kk^-1), split at its barrier.  Step p solves rows j0 = DCW p ..
+// j0 + DCW - 1 (owner; lane = column) after reading the wave's MFMA operand
+// L(16W + (l & 15), j0 + (l >> 4)) for the update that follows the barrier.
 template <int W>
-__device__ inline void inverse_solve(int p, const double (&acc)[16], double (&Lu)[16][DCW], DiagLds& sh, int l) {
+__device__ inline void inverse_solve(int p, const d4 (&U)[4], double& pv, DiagLds& sh, int l) {
   const int j0 = DCW * p;
   double* Rr = sh.R[p & 1];
-#pragma unroll
-  for (int q = 0; q < 16; ++q)
-    if (16 * W + q >= j0 + DCW)
-#pragma unroll
-      for (int s = 0; s < DCW; ++s) Lu[q][s] = sh.L[(j0 + s) * (NB + 2) + 16 * W + q];
+  {
+    const int i = 16 * W + (l & 15);
+    pv = (i >= j0 + DCW) ? sh.L[(j0 + (l >> 4)) * (NB + 2) + i] : 0.0;
+  }
   if (W == j0 / 16) {
+    // the step's rows of the accumulated sums: lane group s holds row j0 + s
+    // (register (j0 & 15) / 4 of each tile); regroup to lane = column
+    const int e = (j0 & 15) / 4;
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) sh.Z[(l >> 4) * NB + 16 * cb + (l & 15)] = (cb <= W) ? U[cb][e] : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     double x[DCW];
 #pragma unroll
     for (int s = 0; s < DCW; ++s) {
-      double t = ((l == j0 + s) ? 1.0 : 0.0) - acc[(j0 & 15) + s];
+      double t = ((l == j0 + s) ? 1.0 : 0.0) - sh.Z[s * NB + l];
 #pragma unroll
       for (int m = 0; m < s; ++m) t = fma(-sh.L[(j0 + m) * (NB + 2) + j0 + s], x[m], t);
       x[s] = t * sh.rinv[j0 + s];
@@ -305,20 +321,19 @@ __device__ inline void inverse_solve(int p, const double (&acc)[16], double (&Lu
   }
 }
 
+// After the barrier: tile cb += L(rows of the wave, step columns) X(step
+// rows, columns of the tile) for the wave's rows below the step (zero P
+// rows elsewhere); tiles right of the wave's rows stay zero (X is lower
+// triangular)
 template <int W>
-__device__ inline void inverse_update(int p, double (&acc)[16], const double (&Lu)[16][DCW], const DiagLds& sh,
-                                      int l) {
+__device__ inline void inverse_update(int p, d4 (&U)[4], double pv, const DiagLds& sh, int l) {
   const int j0 = DCW * p;
+  if (16 * W + 15 < j0 + DCW) return;
   const double* Rr = sh.R[p & 1];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int i = 16 * W + q;
-    if (i >= j0 + DCW) {
-      double t = acc[q];
-#pragma unroll
-      for (int s = 0; s < DCW; ++s) t = fma(Lu[q][s], Rr[s * NB + l], t);
-      acc[q] = t;
-    }
+  for (int cb = 0; cb <= W; ++cb) {
+    const double qv = Rr[(l >> 4) * NB + 16 * cb + (l & 15)];
+    U[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(pv, qv, U[cb], 0, 0, 0);
   }
 }
 
@@ -329,27 +344,27 @@ __device__ inline void inverse_update(int p, double (&acc)[16], const double (&L
 // With the factor given (factor mode), sh.L, sh.diag and sh.rinv are filled
 // before the sweep and the factor waves only keep the barrier count.
 template <int W8>
-__device__ inline bool diag_sweeps(double (&a)[16], DiagLds& sh, int l, bool factor_given) {
+__device__ inline bool diag_sweeps(d4 (&T)[4], DiagLds& sh, int l, bool factor_given) {
   constexpr bool FW = W8 < 4;
   constexpr int W = FW ? W8 : W8 - 4;
   constexpr int NR = NB / DCW;
   bool ok = true;
-  double acc[16];
+  d4 U[4];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) acc[q] = 0.0;
+  for (int cb = 0; cb < 4; ++cb) U[cb] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int r = 0; r < NR + 2; ++r) {
-    double Lu[16][DCW];
+    double pv = 0.0;
     if (FW) {
-      if (r < NR && !factor_given) factor_publish<W>(r, a, sh, l);
+      if (r < NR && !factor_given) factor_publish<W>(r, T, sh, l);
     } else if (r >= 2) {
-      inverse_solve<W>(r - 2, acc, Lu, sh, l);
+      inverse_solve<W>(r - 2, U, pv, sh, l);
     }
     __syncthreads();
     if (FW) {
-      if (r < NR && !factor_given) ok = factor_step<W>(r, a, sh, l) && ok;
+      if (r < NR && !factor_given) ok = factor_step<W>(r, T, sh, l) && ok;
     } else if (r >= 2) {
-      inverse_update<W>(r - 2, acc, Lu, sh, l);
+      inverse_update<W>(r - 2, U, pv, sh, l);
     }
   }
   return ok;
@@ -375,35 +390,43 @@ __global__ __launch_bounds__(DIAG_TPB) void k_potrf_diag(const Slot* __restrict_
   if (tid == 0) sbad = 0;
   const bool given = params[b].mode == MODE_FACTOR;
 
-  double a[16];
-  if (fw) {
+  d4 T[4];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) a[q] = Ab[(size_t)(16 * w + q) * ld + l];
+  for (int rb = 0; rb < 4; ++rb) T[rb] = d4{0.0, 0.0, 0.0, 0.0};
+  if (fw) {
     if (given) {
       // the block is already L_kk: its columns, diagonal and reciprocals for
       // the inverse sweep; a non-positive or NaN diagonal entry is a failure
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int c = 16 * w + q;
-        sh.L[c * (NB + 2) + l] = (l >= c) ? a[q] : 0.0;
+        const double v = Ab[(size_t)c * ld + l];
+        sh.L[c * (NB + 2) + l] = (l >= c) ? v : 0.0;
         if (l == c) {
-          sh.diag[c] = a[q];
-          sh.rinv[c] = 1.0 / a[q];
+          sh.diag[c] = v;
+          sh.rinv[c] = 1.0 / v;
         }
       }
+    } else {
+      // the wave's tiles on and below the diagonal (rows from 16 w)
+#pragma unroll
+      for (int rb = 0; rb < 4; ++rb)
+        if (rb >= w)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) T[rb][e] = Ab[(size_t)(16 * w + (l >> 4) + 4 * e) * ld + 16 * rb + (l & 15)];
     }
   }
   __syncthreads();
   bool ok;
   switch (w) {
-    case 0: ok = diag_sweeps<0>(a, sh, l, given); break;
-    case 1: ok = diag_sweeps<1>(a, sh, l, given); break;
-    case 2: ok = diag_sweeps<2>(a, sh, l, given); break;
-    case 3: ok = diag_sweeps<3>(a, sh, l, given); break;
-    case 4: ok = diag_sweeps<4>(a, sh, l, given); break;
-    case 5: ok = diag_sweeps<5>(a, sh, l, given); break;
-    case 6: ok = diag_sweeps<6>(a, sh, l, given); break;
-    default: ok = diag_sweeps<7>(a, sh, l, given); break;
+    case 0: ok = diag_sweeps<0>(T, sh, l, given); break;
+    case 1: ok = diag_sweeps<1>(T, sh, l, given); break;
+    case 2: ok = diag_sweeps<2>(T, sh, l, given); break;
+    case 3: ok = diag_sweeps<3>(T, sh, l, given); break;
+    case 4: ok = diag_sweeps<4>(T, sh, l, given); break;
+    case 5: ok = diag_sweeps<5>(T, sh, l, given); break;
+    case 6: ok = diag_sweeps<6>(T, sh, l, given); break;
+    default: ok = diag_sweeps<7>(T, sh, l, given); break;
   }
   if (given && fw && l < 16) ok = sh.diag[16 * w + l] > 0.0;
   // every factor lane saw every pivot; in factor mode a lane checks one
@@ -415,12 +438,12 @@ __global__ __launch_bounds__(DIAG_TPB) void k_potrf_diag(const Slot* __restrict_
     return;
   }
   if (fw) {
-    // L_kk -> A (lower, zero above)
+    // L_kk -> A (lower, zero above), from its LDS copy (lane = row)
     double lmax = 0.0;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int c = 16 * w + q;
-      const double v = (l >= c) ? a[q] : 0.0;
+      const double v = sh.L[c * (NB + 2) + l];
       Ab[(size_t)c * ld + l] = v;
       lmax = fmax(lmax, fabs(v));
     }
