@@ -161,6 +161,17 @@ int lkgpu_set_graph(lkgpu_context* ctx, int enable);
  * are the same).  Runs with a timed stage use one stream. */
 int lkgpu_set_streams(lkgpu_context* ctx, int streams);
 
+/* Look-ahead Cholesky for a one-item evaluation of at least 128 64-row
+ * blocks (n > 8128), on (1, the default; LKGPU_LOOKAHEAD=0 turns it off) or
+ * off (0); no reference counterpart.  After panel p's update of panel
+ * p + 1's columns, panel p + 1's diagonal and panel launches run beside
+ * panel p's update of all later columns, which runs on a stream restricted
+ * to all CUs but 32 (LKGPU_LA_RESERVE) so that the chain always finds free
+ * CUs; every tile still takes the panels' updates in panel order, so
+ * results are the same either way.  Such an evaluation is issued without
+ * hipGraph replay (a stream's CU mask does not carry into a graph). */
+int lkgpu_set_lookahead(lkgpu_context* ctx, int enable);
+
 /* Size-bucket launch table (no reference counterpart).  json_text:
  *   {"buckets": [{"max_nblk": B, "panel_blocks": P, "streams": S}, ...]}
  * with B the bucket's largest matrix in 64-row blocks (the last bucket is

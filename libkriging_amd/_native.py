@@ -30,6 +30,7 @@ SIGNATURES = {
     "lkgpu_set_chol_config": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
     "lkgpu_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lkgpu_set_streams": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "lkgpu_set_lookahead": (ctypes.c_int, [_vp, ctypes.c_int]),
     "lkgpu_set_tuning": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "lkgpu_last_parts": (ctypes.c_int, [_vp]),
     "lkgpu_problem_create": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
@@ -130,6 +131,10 @@ class Context:
     def set_streams(self, streams):
         """most streams (1, 2, 4) a batch is split over"""
         check(lib().lkgpu_set_streams(self.h, int(streams)))
+
+    def set_lookahead(self, enable):
+        """look-ahead Cholesky for one-item evaluations of n > 8128"""
+        check(lib().lkgpu_set_lookahead(self.h, int(enable)))
 
     def set_tuning(self, table):
         """size-bucket launch table: a dict {"buckets": [...]}, its JSON

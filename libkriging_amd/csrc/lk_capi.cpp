@@ -156,6 +156,13 @@ int lkgpu_set_streams(lkgpu_context* ctx, int streams) {
   });
 }
 
+int lkgpu_set_lookahead(lkgpu_context* ctx, int enable) {
+  return guarded([&] {
+    if (ctx == nullptr) throw std::invalid_argument("context is NULL");
+    ctx->rt.set_lookahead(enable != 0);
+  });
+}
+
 int lkgpu_set_tuning(lkgpu_context* ctx, const char* json_text) {
   return guarded([&] {
     if (ctx == nullptr) throw std::invalid_argument("context is NULL");

@@ -35,12 +35,22 @@ Runtime::Runtime(int device) : device_(device) {
   if (device < 0 || device >= count) throw std::runtime_error("no HIP device " + std::to_string(device));
   hip_check(hipSetDevice(device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
-  hip_check(hipStreamCreateWithFlags(&la_stream_, hipStreamNonBlocking), "hipStreamCreate");
   for (auto& e : la_ev_) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
   {
     int cus = 0;
     hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
-    la_workgroups_ = 2 * std::max(1, cus);
+    if (const char* e = std::getenv("LKGPU_LA_RESERVE")) la_reserve_ = std::max(0, std::atoi(e));
+    if (la_reserve_ > 0 && la_reserve_ < cus) {
+      // the bulk update's stream on the first cus - la_reserve_ CUs
+      // (n = 8192 Cholesky: 6.44 / 6.44 / 6.60 ms with 24 / 32 / 48 CUs kept
+      // free, 7.00 ms without a mask, 7.05 ms without the look-ahead)
+      std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+      for (int c = 0; c < cus - la_reserve_; ++c) mask[c / 32] |= 1u << (c % 32);
+      hip_check(hipExtStreamCreateWithCUMask(&la_stream_, (uint32_t)mask.size(), mask.data()),
+                "hipExtStreamCreateWithCUMask");
+    } else {
+      hip_check(hipStreamCreateWithFlags(&la_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    }
   }
   if (const char* e = std::getenv("LKGPU_LOOKAHEAD")) lookahead_ = std::atoi(e) != 0;
   if (const char* e = std::getenv("LKGPU_LOOKAHEAD_WG")) la_workgroups_ = std::max(0, std::atoi(e));
@@ -782,8 +792,9 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
             "upload params");
   Part all{0, nslots_, nblk_max_, dmax_};
   apply_tuning(all);
+  const bool la = lookahead_active();
   const std::vector<Step> steps = pipeline_steps(part_batch(all, stream_), want_grad, objective, all.panel_blocks,
-                                                 all.wide, lookahead_ && nslots_ == 1);
+                                                 all.wide, la);
   const auto rng = stage_range(steps, timed_stage_);
   const bool timed = rng.first < rng.second;
   const std::vector<Part> plan = plan_parts(timed, bench);
@@ -810,7 +821,7 @@ void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
     (*bench)["gpu pipeline"] += ms;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-  } else if (use_graph && !noalloc_) {
+  } else if (use_graph && !noalloc_ && !la) {
     if (objective < 0 || objective >= OBJ_COUNT) throw std::invalid_argument("unknown objective");
     GraphEntry& G = graphs_[2 * objective + (want_grad ? 1 : 0)];
     if (G.version != layout_version_ || G.exec == nullptr || G.timed_stage != timed_stage_ || G.split != S) {

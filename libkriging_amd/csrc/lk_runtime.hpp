@@ -189,11 +189,11 @@ class Runtime {
   // load a table ({"buckets": [{"max_nblk", "panel_blocks", "streams"}, ...]})
   void load_tuning_json(const std::string& text);
   int bucket_of(int nblk) const;
-  // look-ahead Cholesky for one-slot batches of at least 16 blocks (off by
-  // default, LKGPU_LOOKAHEAD=1 turns it on): results are the same either
-  // way.  Measured slower at n = 8192 (9.3 vs 7.9 ms): the bulk update's
-  // workgroups keep the chain's panel launches from the CUs.
+  // look-ahead Cholesky for one-slot batches of at least kLookaheadMinBlocks
+  // blocks (on by default; LKGPU_LOOKAHEAD=0 or this call turns it off):
+  // results are the same either way
   void set_lookahead(bool on) { lookahead_ = on; ++layout_version_; }
+  bool lookahead_active() const { return lookahead_ && nslots_ == 1 && nblk_max_ >= kLookaheadMinBlocks; }
   // parts of the last run (slot ranges in bucket order) and their bucket
   int last_parts() const { return last_parts_; }
 
@@ -242,13 +242,20 @@ class Runtime {
   std::vector<BucketTuning> tuning_;  // ascending max_nblk, the last one unbounded
   int last_parts_ = 1;
   // look-ahead Cholesky of a one-slot batch: the next panel's chain on the
-  // main stream beside the bulk trailing update on la_stream_ (persistent,
-  // la_workgroups_ = 2 per CU), ordered by the two events
+  // main stream beside the bulk trailing update on la_stream_, ordered by the
+  // two events.  la_stream_ is restricted to all CUs but la_reserve_ (a CU
+  // mask), which stay free for the chain's launches; the mask is a property
+  // of the stream's hardware queue and does not carry into graph replays,
+  // so the look-ahead pipeline is issued without a graph.
+  // la_workgroups_ > 0 makes the bulk update a persistent launch of that
+  // many workgroups (slower: kept as an option).
   hipStream_t la_stream_ = nullptr;
   hipEvent_t la_ev_[2] = {nullptr, nullptr};
-  int la_workgroups_ = 512;
-  bool lookahead_ = false;
-  static constexpr int kLookaheadMinBlocks = 16;
+  int la_workgroups_ = 0;
+  int la_reserve_ = 32;
+  bool lookahead_ = true;
+  // measured at one slot of n = 1024 .. 8192: a win from 128 blocks on
+  static constexpr int kLookaheadMinBlocks = 128;
   // allocation-free calls: the device scratch array and the host arrays
   // that stand in for the pinned parameter / result tables
   char* scratch_ = nullptr;

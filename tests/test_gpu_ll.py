@@ -282,3 +282,40 @@ def test_bucket_parts_match_one_stream(ctx):
         ctx.set_streams(4)
         for p in pids:
             ctx.drop_problem(p)
+
+
+def test_lookahead_matches_plain(ctx):
+    """the look-ahead Cholesky of a one-item evaluation of 128 blocks (the
+    next panel's chain beside the bulk update on a CU-masked stream, issued
+    without a graph) gives the plain pipeline's results bit for bit: graph
+    and eager runs without it, repeated runs and a timed-stage run with it"""
+    n, d = 8192, 10
+    rng = np.random.default_rng(125)
+    X = rng.uniform(size=(n, d))
+    y = rng.normal(size=n)
+    theta = np.exp(rng.uniform(np.log(0.3), np.log(1.5), size=d))
+    gam = np.concatenate([theta, [1.0 / (1.0 + 1e-6)]])[None, :]
+    pid = ctx.problem(X, y, np.ones((n, 1)), "gauss", "nugget")
+    try:
+        ctx.set_lookahead(False)
+        ll0, g0, st0, inc0 = ctx.ll_grad_batch([pid], gam, True)
+        assert st0[0] == 0
+        ctx.set_graph(False)
+        ll1, g1, _, _ = ctx.ll_grad_batch([pid], gam, True)
+        assert np.array_equal(ll1, ll0) and np.array_equal(g1, g0)
+        ctx.set_graph(True)
+        ctx.set_lookahead(True)
+        for rep in range(2):
+            ll2, g2, st2, inc2 = ctx.ll_grad_batch([pid], gam, True)
+            assert np.array_equal(ll2, ll0) and np.array_equal(g2, g0)
+            assert st2[0] == 0 and inc2[0] == inc0[0]
+        ctx.set_timed_stage("cholesky")
+        ll3, g3, _, _ = ctx.ll_grad_batch([pid], gam, True)
+        ms, count = ctx.take_stage_time()
+        assert count == 1 and ms > 0.0
+        assert np.array_equal(ll3, ll0) and np.array_equal(g3, g0)
+    finally:
+        ctx.set_timed_stage(-1)
+        ctx.set_lookahead(True)
+        ctx.set_graph(True)
+        ctx.drop_problem(pid)
