@@ -975,7 +975,12 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
 // grid (nblk_max (nblk_max + 1) / 2, nslots), 256 threads.  Waves per SIMD:
 // 4 up to 8 dimensions (128 registers, a few values spilled outside the
 // MFMA loop: 0.717 -> 0.680 ms per 64 x n = 1024 launch against 3 waves), 2
-// for 9..16 dimensions (1 before: 9.5 -> 8 ms at n = 8192, d = 10)
+// for 9..16 dimensions (1 before: 9.5 -> 8 ms at n = 8192, d = 10).  Beyond
+// 8 dimensions the epilogue runs its four 16 x 16 blocks as a loop (the
+// sixteen elements are not unrolled there): with the block picked by
+// selects instead of an index into the accumulators, which had moved them
+// to scratch memory, n = 8192, d = 10 went from 7.61 to 3.69 ms per launch
+// (24 -> 50 TFLOP/s) and 64 x n = 1024, d = 10 from 1.55 to 1.06 ms
 // ---------------------------------------------------------------------------
 template <int DT, bool LMP>
 __global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lauum_adjoint(const Slot* __restrict__ slots,
@@ -1052,37 +1057,74 @@ __global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lau
   for (int k = 0; k < DT; ++k) sg[k] = 0.0;
   double a1 = 0.0, a2 = 0.0, dg1 = 0.0, dg2 = 0.0;
 
+  // one element (r, c) of the tile, R^-1 value rinv
+  auto element = [&](int r, int c, double rinv) {
+    const int gi = i0 + r, gj = j0 + c;
+    if (gi >= n || gj > gi) return;
+    if (LMP)
+      for (int a = 0; a < S.p; ++a) rinv = fma(-sG[a * NB + r], sG[PMAX * NB + a * NB + c], rinv);
+    const double xx = sxi[r] * sxj[c];
+    if (gi == gj) {
+      dg1 += xx;
+      dg2 += rinv;
+      return;
+    }
+    double dx[DT];
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
+    for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
+    // rho first, then each g_k recomputed from dx_k while accumulating:
+    // no g[] held across the exp (fewer live registers, more waves)
+    const double rho = kern_rho<DT>(kern, dx, it_);
+    const double Rij = rho * factor;
+    const double wgt = fma(-s2g, rinv, xx);
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
+    for (int k = 0; k < DT; ++k) sg[k] = fma(wgt, Rij * kern_g(kern, dx[k], it_[k]), sg[k]);
+    a1 = fma(xx, rho, a1);
+    a2 = fma(rinv, rho, a2);
+  };
+  if constexpr (DT <= 8) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int r = acc_row(lane, wr, bi), c = acc_col(lane, wc, bj, e);
-        const int gi = i0 + r, gj = j0 + c;
-        double rinv = acc[bi][bj][e];
-        if (gi >= n || gj > gi) continue;
-        if (LMP)
-          for (int a = 0; a < S.p; ++a) rinv = fma(-sG[a * NB + r], sG[PMAX * NB + a * NB + c], rinv);
-        const double xx = sxi[r] * sxj[c];
-        if (gi == gj) {
-          dg1 += xx;
-          dg2 += rinv;
-          continue;
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = acc_row(lane, wr, bi), c = acc_col(lane, wc, bj, e);
+          const int gi = i0 + r, gj = j0 + c;
+          double rinv = acc[bi][bj][e];
+          if (gi >= n || gj > gi) continue;
+          if (LMP)
+            for (int a = 0; a < S.p; ++a) rinv = fma(-sG[a * NB + r], sG[PMAX * NB + a * NB + c], rinv);
+          const double xx = sxi[r] * sxj[c];
+          if (gi == gj) {
+            dg1 += xx;
+            dg2 += rinv;
+            continue;
+          }
+          double dx[DT];
+#pragma unroll
+          for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
+          // rho first, then each g_k recomputed from dx_k while accumulating:
+          // no g[] held across the exp (fewer live registers, more waves)
+          const double rho = kern_rho<DT>(kern, dx, it_);
+          const double Rij = rho * factor;
+          const double wgt = fma(-s2g, rinv, xx);
+#pragma unroll
+          for (int k = 0; k < DT; ++k) sg[k] = fma(wgt, Rij * kern_g(kern, dx[k], it_[k]), sg[k]);
+          a1 = fma(xx, rho, a1);
+          a2 = fma(rinv, rho, a2);
         }
-        double dx[DT];
+  } else {
+    // beyond 8 dimensions the sixteen elements do not unroll: the four
+    // 16 x 16 blocks in a loop, each picked from the accumulators by
+    // selects (a loop index into acc would move it to scratch memory)
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+      const d4 blk = (q == 0) ? acc[0][0] : ((q == 1) ? acc[0][1] : ((q == 2) ? acc[1][0] : acc[1][1]));
 #pragma unroll
-        for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
-        // rho first, then each g_k recomputed from dx_k while accumulating:
-        // no g[] held across the exp (fewer live registers, more waves)
-        const double rho = kern_rho<DT>(kern, dx, it_);
-        const double Rij = rho * factor;
-        const double wgt = fma(-s2g, rinv, xx);
-#pragma unroll
-        for (int k = 0; k < DT; ++k) sg[k] = fma(wgt, Rij * kern_g(kern, dx[k], it_[k]), sg[k]);
-        a1 = fma(xx, rho, a1);
-        a2 = fma(rinv, rho, a2);
-      }
+      for (int e = 0; e < 4; ++e) element(acc_row(lane, wr, q >> 1), acc_col(lane, wc, q & 1, e), blk[e]);
+    }
+  }
 
   // workgroup reduction of the 2 * DT + 4 partials
   __shared__ double sred[4][2 * DT + 4];
