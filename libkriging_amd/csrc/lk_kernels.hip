@@ -1160,9 +1160,11 @@ __global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lau
 // ---------------------------------------------------------------------------
 // (7) gradient (Kriging.cpp:303-326): sums of the tile partials in tile order
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(TPB) void k_finalize_grad(const Slot* __restrict__ slots,
-                                                       const SlotParams* __restrict__ params,
-                                                       SlotOut* __restrict__ out, int nslots) {
+constexpr int FG_TPB = 1024;  // 16 waves
+
+__global__ __launch_bounds__(FG_TPB) void k_finalize_grad(const Slot* __restrict__ slots,
+                                                          const SlotParams* __restrict__ params,
+                                                          SlotOut* __restrict__ out, int nslots) {
   const int b = blockIdx.x;
   if (b >= nslots) return;
   const Slot S = slots[b];
@@ -1171,33 +1173,34 @@ __global__ __launch_bounds__(TPB) void k_finalize_grad(const Slot* __restrict__ 
   if (!slot_live(P, O) || !P.want_grad) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int ntiles = S.nblk * (S.nblk + 1) / 2;
+  constexpr int NW = FG_TPB / 64;
   __shared__ double tot[GRAD_STRIDE];
-  __shared__ double part[TPB / 64][GRAD_STRIDE];
-  // wave w sums the tiles of its quarter, lane = component (a tile's
-  // partials are contiguous: coalesced rows), four running sums in tile
-  // order; the quarters are added in order: deterministic for a given tile
-  // count
+  __shared__ double part[NW][GRAD_STRIDE];
+  // wave w sums the tiles of its sixteenth, lane = component (a tile's
+  // partials are contiguous: coalesced rows), eight running sums in tile
+  // order (eight loads in flight); the sixteenths are added in order:
+  // deterministic for a given tile count
   {
     const double* src = S.red + red_grad_off(S.nblk);
-    const int t0 = (int)(((long long)ntiles * w) / (TPB / 64));
-    const int t1 = (int)(((long long)ntiles * (w + 1)) / (TPB / 64));
+    const int t0 = (int)(((long long)ntiles * w) / NW);
+    const int t1 = (int)(((long long)ntiles * (w + 1)) / NW);
     for (int q = lane; q < GRAD_STRIDE; q += 64) {
-      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      double sm[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) sm[u] = 0.0;
       int t = t0;
-      for (; t + 3 < t1; t += 4) {
-        s0 += src[(size_t)t * GRAD_STRIDE + q];
-        s1 += src[(size_t)(t + 1) * GRAD_STRIDE + q];
-        s2 += src[(size_t)(t + 2) * GRAD_STRIDE + q];
-        s3 += src[(size_t)(t + 3) * GRAD_STRIDE + q];
+      for (; t + 7 < t1; t += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) sm[u] += src[(size_t)(t + u) * GRAD_STRIDE + q];
       }
-      for (; t < t1; ++t) s0 += src[(size_t)t * GRAD_STRIDE + q];
-      part[w][q] = (s0 + s1) + (s2 + s3);
+      for (; t < t1; ++t) sm[0] += src[(size_t)t * GRAD_STRIDE + q];
+      part[w][q] = ((sm[0] + sm[1]) + (sm[2] + sm[3])) + ((sm[4] + sm[5]) + (sm[6] + sm[7]));
     }
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < GRAD_STRIDE; q += TPB) {
+  for (int q = threadIdx.x; q < GRAD_STRIDE; q += FG_TPB) {
     double s = part[0][q];
-    for (int v = 1; v < TPB / 64; ++v) s += part[v][q];
+    for (int v = 1; v < NW; ++v) s += part[v][q];
     tot[q] = s;
   }
   __syncthreads();
@@ -1917,7 +1920,7 @@ static hipError_t launch_lauum_adjoint_t(const Batch& B) {
 hipError_t launch_lauum_adjoint(const Batch& B) { return launch_lauum_adjoint_t<false>(B); }
 hipError_t launch_lmp_adjoint(const Batch& B) { return launch_lauum_adjoint_t<true>(B); }
 hipError_t launch_finalize_grad(const Batch& B) {
-  hipLaunchKernelGGL(k_finalize_grad, dim3(B.nslots), dim3(TPB), 0, B.stream, B.slots, B.params, B.out, B.nslots);
+  hipLaunchKernelGGL(k_finalize_grad, dim3(B.nslots), dim3(FG_TPB), 0, B.stream, B.slots, B.params, B.out, B.nslots);
   return hipGetLastError();
 }
 hipError_t launch_rcond_estimate(const Batch& B) {
