@@ -973,8 +973,9 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
 // place of R^-1 (G = R*^-T F*' L^-1, columns of V from the leave-one-out
 // pass k_loo_g), subtracted from each tile before the contraction.
 // grid (nblk_max (nblk_max + 1) / 2, nslots), 256 threads.  Waves per SIMD:
-// 4 up to 8 dimensions (128 registers, a few values spilled outside the
-// MFMA loop: 0.717 -> 0.680 ms per 64 x n = 1024 launch against 3 waves), 2
+// 4 up to 8 dimensions (128 registers: 0.717 -> 0.680 ms per 64 x n = 1024
+// launch against 3 waves; the diagonal elements' sums are taken before the
+// element loop, which keeps the last few values from spilling), 2
 // for 9..16 dimensions (1 before: 9.5 -> 8 ms at n = 8192, d = 10).  Beyond
 // 8 dimensions the epilogue runs its four 16 x 16 blocks as a loop (the
 // sixteen elements are not unrolled there): with the block picked by
@@ -1055,7 +1056,36 @@ __global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lau
   double sg[DT];
 #pragma unroll
   for (int k = 0; k < DT; ++k) sg[k] = 0.0;
-  double a1 = 0.0, a2 = 0.0, dg1 = 0.0, dg2 = 0.0;
+  double a1 = 0.0, a2 = 0.0;
+  // the diagonal elements' sums first (diagonal tiles only), reduced at once
+  // so that their accumulators are not live across the element loop
+  __shared__ double sred[4][2 * DT + 4];
+  if (it == jt) {
+    double dg1 = 0.0, dg2 = 0.0;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+      for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = acc_row(lane, wr, bi), c = acc_col(lane, wc, bj, e);
+          const int gi = i0 + r, gj = j0 + c;
+          if (gi >= n || gi != gj) continue;
+          double rinv = acc[bi][bj][e];
+          if (LMP)
+            for (int a = 0; a < S.p; ++a) rinv = fma(-sG[a * NB + r], sG[PMAX * NB + a * NB + c], rinv);
+          dg1 += sxi[r] * sxj[c];
+          dg2 += rinv;
+        }
+    const double v2 = wave_sum(dg1), v3 = wave_sum(dg2);
+    if (lane == 0) {
+      sred[w][2 * DT + 2] = v2;
+      sred[w][2 * DT + 3] = v3;
+    }
+  } else if (lane == 0) {
+    sred[w][2 * DT + 2] = 0.0;
+    sred[w][2 * DT + 3] = 0.0;
+  }
 
   // one element (r, c) of the tile, R^-1 value rinv
   auto element = [&](int r, int c, double rinv) {
@@ -1064,11 +1094,7 @@ __global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lau
     if (LMP)
       for (int a = 0; a < S.p; ++a) rinv = fma(-sG[a * NB + r], sG[PMAX * NB + a * NB + c], rinv);
     const double xx = sxi[r] * sxj[c];
-    if (gi == gj) {
-      dg1 += xx;
-      dg2 += rinv;
-      return;
-    }
+    if (gi == gj) return;
     double dx[DT];
 #pragma unroll
     for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
@@ -1096,11 +1122,7 @@ __global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lau
           if (LMP)
             for (int a = 0; a < S.p; ++a) rinv = fma(-sG[a * NB + r], sG[PMAX * NB + a * NB + c], rinv);
           const double xx = sxi[r] * sxj[c];
-          if (gi == gj) {
-            dg1 += xx;
-            dg2 += rinv;
-            continue;
-          }
+          if (gi == gj) continue;
           double dx[DT];
 #pragma unroll
           for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
@@ -1127,7 +1149,6 @@ __global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lau
   }
 
   // workgroup reduction of the 2 * DT + 4 partials
-  __shared__ double sred[4][2 * DT + 4];
   double* dst = S.red + red_grad_off(S.nblk) + (size_t)ts.tile * GRAD_STRIDE;
 #pragma unroll
   for (int k = 0; k < DT; ++k) {
@@ -1138,12 +1159,10 @@ __global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lau
     }
   }
   {
-    const double v0 = wave_sum(a1), v1 = wave_sum(a2), v2 = wave_sum(dg1), v3 = wave_sum(dg2);
+    const double v0 = wave_sum(a1), v1 = wave_sum(a2);
     if (lane == 0) {
       sred[w][2 * DT + 0] = v0;
       sred[w][2 * DT + 1] = v1;
-      sred[w][2 * DT + 2] = v2;
-      sred[w][2 * DT + 3] = v3;
     }
   }
   __syncthreads();
