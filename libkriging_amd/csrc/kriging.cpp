@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <limits>
 #include <map>
 #include <mutex>
@@ -765,7 +766,22 @@ void Kriging::run_fit_jobs(const std::vector<FitJob*>& jobs_in) {
   }
   std::vector<EvalResult> cres;
   rt->evaluate(creq, false, cres);
-  for (size_t j = 0; j < jobs.size(); ++j) jobs[j]->m->commit_apply(jobs[j]->theta, jobs[j]->alpha, (int)j, cres[j]);
+  // every model's copies first, one synchronisation, then the host state;
+  // a model that failed commits none of its state (the ones before it are
+  // committed, as one commit after the other would leave them)
+  size_t ncopied = 0;
+  std::exception_ptr err;
+  for (; ncopied < jobs.size(); ++ncopied) {
+    try {
+      jobs[ncopied]->m->commit_copy((int)ncopied, cres[ncopied]);
+    } catch (...) {
+      err = std::current_exception();
+      break;
+    }
+  }
+  hip_check(hipStreamSynchronize(rt->stream()), "sync");
+  for (size_t j = 0; j < ncopied; ++j) jobs[j]->m->commit_finish(jobs[j]->theta, jobs[j]->alpha, (int)j, cres[j]);
+  if (err) std::rethrow_exception(err);
 }
 
 // ---------------------------------------------------------------------------
@@ -1063,38 +1079,56 @@ void Kriging::commit(const std::vector<double>& theta, double alpha) {
   commit_apply(theta, alpha, 0, res[0]);
 }
 
+// a model buffer of `bytes`, reallocated only when its size changes
+static void ensure_model_buffer(double*& ptr, size_t& cap, size_t bytes) {
+  if (ptr != nullptr && cap == bytes) return;
+  if (ptr != nullptr) hipFree(ptr);
+  ptr = nullptr;
+  cap = 0;
+  hip_check(hipMalloc(&ptr, bytes), "hipMalloc model");
+  cap = bytes;
+}
+
 void Kriging::commit_apply(const std::vector<double>& theta, double alpha, int slot, const EvalResult& r) {
+  commit_copy(slot, r);
+  hip_check(hipStreamSynchronize(rt_->stream()), "sync");
+  commit_finish(theta, alpha, slot, r);
+}
+
+// the device copies of slot `slot`'s factors into the model's buffers,
+// enqueued on the runtime's stream (no synchronisation)
+void Kriging::commit_copy(int slot, const EvalResult& r) {
   const int p = p_;
   if (r.status != EVAL_OK) throw std::runtime_error(chol_failure_message(r.status, r.inc, rt_->chol.num_nugget));
-  theta_ = theta;
   const Runtime::SlotView v = rt_->slot_view(slot);
   npad_ = v.npad;
   hip_check(hipSetDevice(device_), "hipSetDevice");
-  if (d_L_) hipFree(d_L_);
-  if (d_Linv_) hipFree(d_Linv_);
-  if (d_vec_) hipFree(d_vec_);
-  d_L_ = nullptr;
-  d_Linv_ = nullptr;
-  d_vec_ = nullptr;
-  hip_check(hipMalloc(&d_L_, sizeof(double) * (size_t)n_ * n_), "hipMalloc model");
+  ensure_model_buffer(d_L_, cap_L_, sizeof(double) * (size_t)n_ * n_);
+  ensure_model_buffer(d_Linv_, cap_Linv_, sizeof(double) * (size_t)npad_ * npad_);
+  ensure_model_buffer(d_vec_, cap_vec_, sizeof(double) * (size_t)npad_ * (p + 1));
   hip_check(hipMemcpy2DAsync(d_L_, sizeof(double) * n_, v.A, sizeof(double) * npad_, sizeof(double) * n_, n_,
                              hipMemcpyDeviceToDevice, rt_->stream()),
             "copy model");
-  hip_check(hipMalloc(&d_Linv_, sizeof(double) * (size_t)npad_ * npad_), "hipMalloc model");
-  hip_check(hipMalloc(&d_vec_, sizeof(double) * (size_t)npad_ * (p + 1)), "hipMalloc model");
   hip_check(hipMemcpyAsync(d_Linv_, v.W, sizeof(double) * (size_t)npad_ * npad_, hipMemcpyDeviceToDevice,
                            rt_->stream()),
             "copy model");
   if (p > 0)
     hip_check(hipMemcpyAsync(d_vec_, v.V, sizeof(double) * (size_t)npad_ * p, hipMemcpyDeviceToDevice, rt_->stream()),
               "copy model");
+  if (est_beta_)
+    hip_check(hipMemcpyAsync(d_vec_ + (size_t)npad_ * p, v.V + (size_t)(p + 2) * npad_, sizeof(double) * npad_,
+                             hipMemcpyDeviceToDevice, rt_->stream()),
+              "copy model");
+}
+
+// the host state of the commit, once commit_copy's copies are done
+void Kriging::commit_finish(const std::vector<double>& theta, double alpha, int slot, const EvalResult& r) {
+  const int p = p_;
+  theta_ = theta;
   const double sse = r.sse;
   std::vector<double> bhat(r.beta, r.beta + p);
   if (est_beta_) {
     beta_ = bhat;
-    hip_check(hipMemcpyAsync(d_vec_ + (size_t)npad_ * p, v.V + (size_t)(p + 2) * npad_, sizeof(double) * npad_,
-                             hipMemcpyDeviceToDevice, rt_->stream()),
-              "copy model");
   } else {
     // z = y* - M beta (Kriging.cpp:2170-2172)
     std::vector<double> ystar(n_), M((size_t)n_ * std::max(p, 1)), z(npad_, 0.0);
@@ -1105,14 +1139,11 @@ void Kriging::commit_apply(const std::vector<double>& theta, double alpha, int s
       for (int a = 0; a < p; ++a) s -= M[i + (size_t)a * n_] * beta_[a];
       z[i] = s;
     }
-    // z is local: copy before it goes out of scope (the stream is idle here:
-    // read_slot synchronised it)
     hip_check(hipMemcpy(d_vec_ + (size_t)npad_ * p, z.data(), sizeof(double) * npad_, hipMemcpyHostToDevice),
               "upload z");
   }
   Rstar_.assign((size_t)p * p, 0.0);
   if (p > 0) rt_->read_slot(slot, 6, Rstar_.data());
-  hip_check(hipStreamSynchronize(rt_->stream()), "sync");
   const int n = n_;
   const double total_var = sse / n;
   if (noise_ == NoiseModel::Nugget) {

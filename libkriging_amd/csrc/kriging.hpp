@@ -176,6 +176,10 @@ class Kriging {
   // request, then the copy of slot `slot`'s factors once it is evaluated
   EvalRequest commit_request(const std::vector<double>& gamma) const;
   void commit_apply(const std::vector<double>& theta, double alpha, int slot, const EvalResult& r);
+  // commit_apply in two halves, so that many models' copies go out before
+  // one synchronisation: the device copies (enqueued), then the host state
+  void commit_copy(int slot, const EvalResult& r);
+  void commit_finish(const std::vector<double>& theta, double alpha, int slot, const EvalResult& r);
   void commit(const std::vector<double>& theta, double alpha);
   void theta_bounds(std::vector<double>& lo, std::vector<double>& up);
 
@@ -224,6 +228,7 @@ class Kriging {
   double* d_L_ = nullptr;
   double* d_Linv_ = nullptr;
   double* d_vec_ = nullptr;  // npad x (p + 1): [M | z]
+  size_t cap_L_ = 0, cap_Linv_ = 0, cap_vec_ = 0;  // their sizes in bytes (reused when unchanged)
   int npad_ = 0;
   std::vector<double> Rstar_;  // p x p upper
 
