@@ -119,6 +119,8 @@ Kriging::~Kriging() {
 // fit_setup_impl (KrigingImpl.cpp:764-841): normalisation, trend matrix
 void Kriging::setup(const double* y, const double* X, int n, int d, RegModel regmodel, bool normalize) {
   if (n <= 0 || d <= 0) throw std::invalid_argument("empty design");
+  const int n_old = n_, d_old = d_;
+  const RegModel regmodel_old = regmodel_;
   n_ = n;
   d_ = d;
   regmodel_ = regmodel;
@@ -145,12 +147,16 @@ void Kriging::setup(const double* y, const double* X, int n, int d, RegModel reg
     centerY_ = mn;
     scaleY_ = mx - mn;
   }
-  X_.resize((size_t)n * d);
-  y_.resize(n);
+  std::vector<double> Xn((size_t)n * d), yn(n);
   for (int k = 0; k < d; ++k)
-    for (int i = 0; i < n; ++i) X_[i + (size_t)k * n] = (X[i + (size_t)k * n] - centerX_[k]) / scaleX_[k];
-  for (int i = 0; i < n; ++i) y_[i] = (y[i] - centerY_) / scaleY_;
-  upload_data();
+    for (int i = 0; i < n; ++i) Xn[i + (size_t)k * n] = (X[i + (size_t)k * n] - centerX_[k]) / scaleX_[k];
+  for (int i = 0; i < n; ++i) yn[i] = (y[i] - centerY_) / scaleY_;
+  // the device problem already holds these data (a re-fit on the same
+  // design, e.g. NestedKriging's common-prior pass): keep it
+  const bool same = problem_ >= 0 && n == n_old && d == d_old && regmodel == regmodel_old && Xn == X_ && yn == y_;
+  X_.swap(Xn);
+  y_.swap(yn);
+  if (!same) upload_data();
   is_empty_ = true;
 }
 
