@@ -556,7 +556,9 @@ __global__ __launch_bounds__(TPB) void k_potrf_panel(const Slot* __restrict__ sl
 // update reaches only the panel's own columns (K = 64); the rest of the
 // matrix is updated once per panel with K = 64 P, so the trailing matrix is
 // read and written 1 / P as often.
-// grid (tiles of the widest slot, nslots), 256 threads
+// grid (tiles of the widest slot, nslots), 256 threads, four waves per SIMD
+// (128 registers, no spill: the 64 x n = 1024 batch's Cholesky 1.035 ->
+// 1.021 ms against three)
 // ---------------------------------------------------------------------------
 // tile number `tile` of the update's tile set -> (r, c); false when out of range
 __device__ inline bool update_tile_rc(int tile, int nb, int c0, int c1, int& r, int& c) {
@@ -591,7 +593,7 @@ __device__ inline void update_tile(const Slot& S, int r, int c, int kb, int ke, 
   acc_store(acc, C, ld, lane, wr, wc);
 }
 
-__global__ __launch_bounds__(TPB) void k_potrf_update(const Slot* __restrict__ slots,
+__global__ __launch_bounds__(TPB, 4) void k_potrf_update(const Slot* __restrict__ slots,
                                                       const SlotParams* __restrict__ params,
                                                       SlotOut* __restrict__ out, int kb, int ke, int c0, int c1) {
   const TileSlot ts = tile_slot();
@@ -671,9 +673,10 @@ __global__ __launch_bounds__(TPB) void k_potrf_update_persistent(const Slot* __r
 // aligned pair (top = 2ps, bot = top + s):
 //   step 0:  T = L(bot, top) * Linv(top, top)          -> W upper (transposed)
 //   step 1:  Linv(bot, top) = -Linv(bot, bot) * T      -> W lower
-// grid (npairs_max * s * s, nslots), 256 threads
+// grid (npairs_max * s * s, nslots), 256 threads, four waves per SIMD (124
+// registers, no spill: 64 x n = 1024 L^-1 0.664 -> 0.651 ms against three)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(TPB) void k_trtri(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
+__global__ __launch_bounds__(TPB, 4) void k_trtri(const Slot* __restrict__ slots, const SlotParams* __restrict__ params,
                                                SlotOut* __restrict__ out, int s, int step) {
   const TileSlot ts = tile_slot();
   const int b = ts.slot;
