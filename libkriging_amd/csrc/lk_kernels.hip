@@ -1668,8 +1668,27 @@ __global__ __launch_bounds__(TPB) void k_predict_cov(PredictArgs a) {
         for (int q = 0; q < a.p; ++q) ec = fma(a.ecirc[(size_t)q * m + gi], a.ecirc[(size_t)q * m + gj], ec);
         const double v = acc[bi][bj][e] + rnn + ec;
         a.cov[(size_t)gj * m + gi] = v;
-        if (ti != tj) a.cov[(size_t)gi * m + gj] = v;
+        acc[bi][bj][e] = v;
       }
+  if (ti == tj) return;
+  // the transpose of an off-diagonal tile, staged through LDS so that its
+  // columns are stored with 64 consecutive lanes (coalesced)
+  static_assert(4 * KC * LDSS >= NB * (NB + 1), "a 64 x 65 tile fits the staging buffer");
+  double* sT = sbuf;
+  __syncthreads();  // every wave is done with sXi / sXj
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        sT[acc_row(lane, wr, bi) * (NB + 1) + acc_col(lane, wc, bj, e)] = acc[bi][bj][e];
+  __syncthreads();
+  const int gj = j0 + lane;
+  for (int r = w * (NB / 4); r < (w + 1) * (NB / 4); ++r) {
+    const int gi = i0 + r;
+    if (gi < m && gj < m) a.cov[(size_t)gi * m + gj] = sT[r * (NB + 1) + lane];
+  }
 }
 
 // d ln k / d dx_k (Covariance.cpp:52-63, 90-101, 132-146, 181-198), dx = x_new - x_obs
