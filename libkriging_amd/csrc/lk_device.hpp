@@ -412,25 +412,33 @@ __device__ inline void acc_store_t(const d4 (&acc)[2][2], double* __restrict__ C
 constexpr double SQRT3 = 1.7320508075688772;
 constexpr double SQRT5 = 2.2360679774997896;
 
-// rho(dx) for the kernel; it[k] = 1 / theta_k (entries k >= d hold dx = 0)
+// rho(dx) for the kernel; it[k] = 1 / theta_k.  Coordinates from dlim on
+// are skipped (a uniform branch, no work); those below DT and beyond the
+// design's d must then hold dx = 0 (they add exact zeros: the result is the
+// same either way)
 template <int DT>
-__device__ inline double kern_rho(int kern, const double* dx, const double* it) {
+__device__ inline double kern_rho(int kern, const double* dx, const double* it, int dlim = DT) {
   double s = 0.0;
   if (kern == K_GAUSS) {
 #pragma unroll
     for (int k = 0; k < DT; ++k) {
+      if (k >= dlim) break;
       const double v = dx[k] * it[k];
       s = fma(v, v, s);
     }
     return exp(-0.5 * s);
   } else if (kern == K_EXP) {
 #pragma unroll
-    for (int k = 0; k < DT; ++k) s += fabs(dx[k] * it[k]);
+    for (int k = 0; k < DT; ++k) {
+      if (k >= dlim) break;
+      s += fabs(dx[k] * it[k]);
+    }
     return exp(-s);
   } else if (kern == K_MATERN32) {
     double pr = 1.0;
 #pragma unroll
     for (int k = 0; k < DT; ++k) {
+      if (k >= dlim) break;
       const double u = SQRT3 * fabs(dx[k] * it[k]);
       s += u;
       pr *= 1.0 + u;
@@ -440,6 +448,7 @@ __device__ inline double kern_rho(int kern, const double* dx, const double* it) 
     double pr = 1.0;
 #pragma unroll
     for (int k = 0; k < DT; ++k) {
+      if (k >= dlim) break;
       const double u = SQRT5 * fabs(dx[k] * it[k]);
       s += u;
       pr *= 1.0 + u + (u * u) * (1.0 / 3.0);

@@ -986,10 +986,12 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
 // sixteen elements are not unrolled there): with the block picked by
 // selects instead of an index into the accumulators, which had moved them
 // to scratch memory, n = 8192, d = 10 went from 7.61 to 3.69 ms per launch
-// (24 -> 50 TFLOP/s) and 64 x n = 1024, d = 10 from 1.55 to 1.06 ms
+// (24 -> 50 TFLOP/s) and 64 x n = 1024, d = 10 from 1.55 to 1.06 ms.  9..12
+// dimensions use their own instance, which skips the padded coordinates
+// (166 registers, three waves: d = 10 at 3.53 / 0.96 ms)
 // ---------------------------------------------------------------------------
 template <int DT, bool LMP>
-__global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lauum_adjoint(const Slot* __restrict__ slots,
+__global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))) void k_lauum_adjoint(const Slot* __restrict__ slots,
                                                        const SlotParams* __restrict__ params,
                                                        SlotOut* __restrict__ out) {
   const TileSlot ts = tile_slot();
@@ -1100,16 +1102,22 @@ __global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lau
       for (int a = 0; a < S.p; ++a) rinv = fma(-sG[a * NB + r], sG[PMAX * NB + a * NB + c], rinv);
     const double xx = sxi[r] * sxj[c];
     if (gi == gj) return;
+    // in the 12-dimension instance the coordinates from d on are skipped
+    // (uniform branches): fewer registers, three waves per SIMD
+    constexpr bool kSkip = DT == 12;
     double dx[DT];
 #pragma unroll
     for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
     // rho first, then each g_k recomputed from dx_k while accumulating:
     // no g[] held across the exp (fewer live registers, more waves)
-    const double rho = kern_rho<DT>(kern, dx, it_);
+    const double rho = kern_rho<DT>(kern, dx, it_, kSkip ? d : DT);
     const double Rij = rho * factor;
     const double wgt = fma(-s2g, rinv, xx);
 #pragma unroll
-    for (int k = 0; k < DT; ++k) sg[k] = fma(wgt, Rij * kern_g(kern, dx[k], it_[k]), sg[k]);
+    for (int k = 0; k < DT; ++k) {
+      if (kSkip && k >= d) break;
+      sg[k] = fma(wgt, Rij * kern_g(kern, dx[k], it_[k]), sg[k]);
+    }
     a1 = fma(xx, rho, a1);
     a2 = fma(rinv, rho, a2);
   };
@@ -1132,7 +1140,9 @@ __global__ __launch_bounds__(TPB, (DT == 16 ? 2 : (DT <= 8 ? 4 : 1))) void k_lau
 #pragma unroll
           for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
           // rho first, then each g_k recomputed from dx_k while accumulating:
-          // no g[] held across the exp (fewer live registers, more waves)
+          // no g[] held across the exp (fewer live registers, more waves);
+          // the padded coordinates are computed (zeros): skipping them by
+          // branches here makes the unrolled elements spill
           const double rho = kern_rho<DT>(kern, dx, it_);
           const double Rij = rho * factor;
           const double wgt = fma(-s2g, rinv, xx);
@@ -1954,6 +1964,8 @@ static hipError_t launch_lauum_adjoint_t(const Batch& B) {
     hipLaunchKernelGGL((k_lauum_adjoint<4, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   else if (dmax_batch <= 8)
     hipLaunchKernelGGL((k_lauum_adjoint<8, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  else if (dmax_batch <= 12)
+    hipLaunchKernelGGL((k_lauum_adjoint<12, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   else if (dmax_batch <= 16)
     hipLaunchKernelGGL((k_lauum_adjoint<16, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   else
