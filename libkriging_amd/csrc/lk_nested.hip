@@ -225,7 +225,8 @@ __global__ __launch_bounds__(TPB, 3) void k_nk_pair(NkArgs a) {
         const int kk = (tid >> 6) + 4 * r;
 #pragma unroll
         for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? sXa[k * NB + ii] - sXb[k * NK_PKC + kk] : 0.0;
-        sA[kk * LDSS + ii] = kern_rho<DT>(a.kernel, dx, it);
+        // the padded coordinates (d .. DT - 1) are skipped
+        sA[kk * LDSS + ii] = kern_rho<DT>(a.kernel, dx, it, d);
       }
       __syncthreads();
 #pragma unroll
