@@ -119,7 +119,7 @@ __global__ __launch_bounds__(TPB) void k_build(const Slot* __restrict__ slots, c
     } else {
 #pragma unroll
       for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? sXj[k * NB + jj] - sXi[k * NB + ii] : 0.0;
-      v = kern_rho<DT>(S.kernel, dx, it) * factor;
+      v = kern_rho<DT>(S.kernel, dx, it, DT <= 16 ? d : DT) * factor;  // padded coordinates skipped
     }
     S.A[(size_t)gj * S.npad + gi] = v;
   }
