@@ -988,7 +988,9 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
 // to scratch memory, n = 8192, d = 10 went from 7.61 to 3.69 ms per launch
 // (24 -> 50 TFLOP/s) and 64 x n = 1024, d = 10 from 1.55 to 1.06 ms.  9..12
 // dimensions use their own instance, which skips the padded coordinates
-// (166 registers, three waves: d = 10 at 3.53 / 0.96 ms)
+// (166 registers, three waves: d = 10 at 3.53 / 0.96 ms); up to 6
+// dimensions the 6-dimension instance (64 x n = 1024, d = 6: 0.667 -> 0.630
+// ms against the 8-dimension one, which computes two padded coordinates)
 // ---------------------------------------------------------------------------
 template <int DT, bool LMP>
 __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))) void k_lauum_adjoint(const Slot* __restrict__ slots,
@@ -1112,11 +1114,11 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
     // no g[] held across the exp (fewer live registers, more waves)
     const double rho = kern_rho<DT>(kern, dx, it_, kSkip ? d : DT);
     const double Rij = rho * factor;
-    const double wgt = fma(-s2g, rinv, xx);
+    const double wf = fma(-s2g, rinv, xx) * Rij;  // one product per element, not per k
 #pragma unroll
     for (int k = 0; k < DT; ++k) {
       if (kSkip && k >= d) break;
-      sg[k] = fma(wgt, Rij * kern_g(kern, dx[k], it_[k]), sg[k]);
+      sg[k] = fma(wf, kern_g(kern, dx[k], it_[k]), sg[k]);
     }
     a1 = fma(xx, rho, a1);
     a2 = fma(rinv, rho, a2);
@@ -1145,9 +1147,9 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
           // branches here makes the unrolled elements spill
           const double rho = kern_rho<DT>(kern, dx, it_);
           const double Rij = rho * factor;
-          const double wgt = fma(-s2g, rinv, xx);
+          const double wf = fma(-s2g, rinv, xx) * Rij;  // one product per element, not per k
 #pragma unroll
-          for (int k = 0; k < DT; ++k) sg[k] = fma(wgt, Rij * kern_g(kern, dx[k], it_[k]), sg[k]);
+          for (int k = 0; k < DT; ++k) sg[k] = fma(wf, kern_g(kern, dx[k], it_[k]), sg[k]);
           a1 = fma(xx, rho, a1);
           a2 = fma(rinv, rho, a2);
         }
@@ -1962,6 +1964,8 @@ static hipError_t launch_lauum_adjoint_t(const Batch& B) {
   const int dmax_batch = B.dmax;
   if (dmax_batch <= 4)
     hipLaunchKernelGGL((k_lauum_adjoint<4, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
+  else if (dmax_batch <= 6)
+    hipLaunchKernelGGL((k_lauum_adjoint<6, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   else if (dmax_batch <= 8)
     hipLaunchKernelGGL((k_lauum_adjoint<8, LMP>), grid, dim3(TPB), 0, B.stream, B.slots, B.params, B.out);
   else if (dmax_batch <= 12)
