@@ -1025,10 +1025,12 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
   double* sXj = sXi + DT * NB;
   __shared__ double sxi[NB], sxj[NB];
   const double* xv = S.V + (size_t)(S.p + 3) * ld;
-  for (int q = tid; q < d * NB; q += TPB) {
+  // all DT coordinate rows, zero from d on: the padded coordinate
+  // differences are then zero without a select per element
+  for (int q = tid; q < DT * NB; q += TPB) {
     const int k = q / NB, r = q % NB;
-    sXi[q] = (i0 + r < n) ? S.X[(size_t)k * n + i0 + r] : 0.0;
-    sXj[q] = (j0 + r < n) ? S.X[(size_t)k * n + j0 + r] : 0.0;
+    sXi[q] = (k < d && i0 + r < n) ? S.X[(size_t)k * n + i0 + r] : 0.0;
+    sXj[q] = (k < d && j0 + r < n) ? S.X[(size_t)k * n + j0 + r] : 0.0;
   }
   if (tid < NB) {
     sxi[tid] = xv[i0 + tid];
@@ -1096,6 +1098,10 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
     sred[w][2 * DT + 3] = 0.0;
   }
 
+  // unit inverse length-scales for kern_rho on scaled differences (folded)
+  [[maybe_unused]] double kOnes[DT];
+#pragma unroll
+  for (int k = 0; k < DT; ++k) kOnes[k] = 1.0;
   // one element (r, c) of the tile, R^-1 value rinv
   auto element = [&](int r, int c, double rinv) {
     const int gi = i0 + r, gj = j0 + c;
@@ -1109,7 +1115,7 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
     constexpr bool kSkip = DT == 12;
     double dx[DT];
 #pragma unroll
-    for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
+    for (int k = 0; k < DT; ++k) dx[k] = sXj[k * NB + c] - sXi[k * NB + r];
     // rho first, then each g_k recomputed from dx_k while accumulating:
     // no g[] held across the exp (fewer live registers, more waves)
     const double rho = kern_rho<DT>(kern, dx, it_, kSkip ? d : DT);
@@ -1138,18 +1144,22 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
             for (int a = 0; a < S.p; ++a) rinv = fma(-sG[a * NB + r], sG[PMAX * NB + a * NB + c], rinv);
           const double xx = sxi[r] * sxj[c];
           if (gi == gj) continue;
+          // scaled coordinate differences dx_k / theta_k (the products
+          // kern_rho forms): rho from them, and g_k = d ln rho / d theta_k
+          // as g(dx_k / theta_k; 1) / theta_k, the 1 / theta_k applied once
+          // per tile (one product per coordinate fewer)
           double dx[DT];
 #pragma unroll
-          for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
+          for (int k = 0; k < DT; ++k) dx[k] = (sXj[k * NB + c] - sXi[k * NB + r]) * it_[k];
           // rho first, then each g_k recomputed from dx_k while accumulating:
           // no g[] held across the exp (fewer live registers, more waves);
           // the padded coordinates are computed (zeros): skipping them by
           // branches here makes the unrolled elements spill
-          const double rho = kern_rho<DT>(kern, dx, it_);
+          const double rho = kern_rho<DT>(kern, dx, kOnes);
           const double Rij = rho * factor;
           const double wf = fma(-s2g, rinv, xx) * Rij;  // one product per element, not per k
 #pragma unroll
-          for (int k = 0; k < DT; ++k) sg[k] = fma(wf, kern_g(kern, dx[k], it_[k]), sg[k]);
+          for (int k = 0; k < DT; ++k) sg[k] = fma(wf, kern_g(kern, dx[k], 1.0), sg[k]);
           a1 = fma(xx, rho, a1);
           a2 = fma(rinv, rho, a2);
         }
@@ -1165,6 +1175,11 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
     }
   }
 
+  // up to 8 dimensions g_k carried 1 / theta_k as a common factor
+  if constexpr (DT <= 8) {
+#pragma unroll
+    for (int k = 0; k < DT; ++k) sg[k] *= it_[k];
+  }
   // workgroup reduction of the 2 * DT + 4 partials
   double* dst = S.red + red_grad_off(S.nblk) + (size_t)ts.tile * GRAD_STRIDE;
 #pragma unroll
