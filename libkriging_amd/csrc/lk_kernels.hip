@@ -1098,6 +1098,8 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
     sred[w][2 * DT + 3] = 0.0;
   }
 
+  // the alpha direction's sums only for a nugget model whose gamma has alpha
+  const bool need_alpha = S.noise == NOISE_NUGGET && P.gamma_has_alpha;
   // unit inverse length-scales for kern_rho on scaled differences (folded)
   [[maybe_unused]] double kOnes[DT];
 #pragma unroll
@@ -1160,8 +1162,10 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
           const double wf = fma(-s2g, rinv, xx) * Rij;  // one product per element, not per k
 #pragma unroll
           for (int k = 0; k < DT; ++k) sg[k] = fma(wf, kern_g(kern, dx[k], 1.0), sg[k]);
-          a1 = fma(xx, rho, a1);
-          a2 = fma(rinv, rho, a2);
+          if (need_alpha) {
+            a1 = fma(xx, rho, a1);
+            a2 = fma(rinv, rho, a2);
+          }
         }
   } else {
     // beyond 8 dimensions the sixteen elements do not unroll: the four
