@@ -20,6 +20,8 @@
 // reduced in registers in a fixed order (deterministic, no atomics).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <math.h>
 
 #include "lk_device.hpp"
@@ -152,7 +154,7 @@ constexpr int NK_QW = NK_QT / 4;  // points per wave
 constexpr int NK_QB = NK_QW / 16;  // 16-point MFMA blocks per wave
 constexpr int NK_SB = NK_QT + 1;  // W_j chunk row stride (odd: transposed stores conflict-free)
 
-template <int DT>
+template <int DT, int KT>
 __global__ __launch_bounds__(TPB, 3) void k_nk_pair(NkArgs a) {
   const int2 pr = a.pairs[blockIdx.x];
   const int t0 = blockIdx.y * NK_QT;
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(TPB, 3) void k_nk_pair(NkArgs a) {
 #pragma unroll
         for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? sXa[k * NB + ii] - sXb[k * NK_PKC + kk] : 0.0;
         // the padded coordinates (d .. DT - 1) are skipped
-        sA[kk * LDSS + ii] = kern_rho<DT>(a.kernel, dx, it, d);
+        sA[kk * LDSS + ii] = kern_rho<DT>(KT, dx, it, d);
       }
       __syncthreads();
 #pragma unroll
@@ -482,14 +484,25 @@ hipError_t launch_nk_pairs(const NkArgs& a, hipStream_t stream) {
   if (a.npairs <= 0) return hipSuccess;
   if (a.qpad % NK_QT != 0) return hipErrorInvalidValue;
   const dim3 grid(a.npairs, a.qpad / NK_QT);
+  // one instance per (dimension bound, covariance kernel): the kernel-value
+  // generation runs one kernel's code, not a branch per value
+  auto go = [&](auto dtag) {
+    constexpr int DT = decltype(dtag)::value;
+    switch (a.kernel) {
+      case K_GAUSS: hipLaunchKernelGGL((k_nk_pair<DT, K_GAUSS>), grid, dim3(TPB), 0, stream, a); break;
+      case K_EXP: hipLaunchKernelGGL((k_nk_pair<DT, K_EXP>), grid, dim3(TPB), 0, stream, a); break;
+      case K_MATERN32: hipLaunchKernelGGL((k_nk_pair<DT, K_MATERN32>), grid, dim3(TPB), 0, stream, a); break;
+      default: hipLaunchKernelGGL((k_nk_pair<DT, K_MATERN52>), grid, dim3(TPB), 0, stream, a); break;
+    }
+  };
   if (a.d <= 4)
-    hipLaunchKernelGGL(k_nk_pair<4>, grid, dim3(TPB), 0, stream, a);
+    go(std::integral_constant<int, 4>{});
   else if (a.d <= 8)
-    hipLaunchKernelGGL(k_nk_pair<8>, grid, dim3(TPB), 0, stream, a);
+    go(std::integral_constant<int, 8>{});
   else if (a.d <= 16)
-    hipLaunchKernelGGL(k_nk_pair<16>, grid, dim3(TPB), 0, stream, a);
+    go(std::integral_constant<int, 16>{});
   else
-    hipLaunchKernelGGL(k_nk_pair<32>, grid, dim3(TPB), 0, stream, a);
+    go(std::integral_constant<int, 32>{});
   return hipGetLastError();
 }
 
