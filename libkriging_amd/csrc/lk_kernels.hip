@@ -1117,7 +1117,8 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
 #pragma unroll
   for (int k = 0; k < DT; ++k) kOnes[k] = 1.0;
   // one element (r, c) of the tile, R^-1 value rinv
-  auto element = [&](int r, int c, double rinv) {
+  auto element = [&](auto ktag, int r, int c, double rinv) __attribute__((always_inline)) {
+    constexpr int KT = decltype(ktag)::value;
     const int gi = i0 + r, gj = j0 + c;
     if (gi >= n || gj > gi) return;
     if (LMP)
@@ -1132,13 +1133,13 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
     for (int k = 0; k < DT; ++k) dx[k] = sXj[k * NB + c] - sXi[k * NB + r];
     // rho first, then each g_k recomputed from dx_k while accumulating:
     // no g[] held across the exp (fewer live registers, more waves)
-    const double rho = kern_rho<DT>(kern, dx, it_, kSkip ? d : DT);
+    const double rho = kern_rho<DT>(KT, dx, it_, kSkip ? d : DT);
     const double Rij = rho * factor;
     const double wf = fma(-s2g, rinv, xx) * Rij;  // one product per element, not per k
 #pragma unroll
     for (int k = 0; k < DT; ++k) {
       if (kSkip && k >= d) break;
-      sg[k] = fma(wf, kern_g(kern, dx[k], it_[k]), sg[k]);
+      sg[k] = fma(wf, kern_g(KT, dx[k], it_[k]), sg[k]);
     }
     a1 = fma(xx, rho, a1);
     a2 = fma(rinv, rho, a2);
@@ -1195,11 +1196,20 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
     // beyond 8 dimensions the sixteen elements do not unroll: the four
     // 16 x 16 blocks in a loop, each picked from the accumulators by
     // selects (a loop index into acc would move it to scratch memory)
+    // (specialised per covariance kernel, as above)
+    auto blocks = [&](auto ktag) __attribute__((always_inline)) {
 #pragma unroll 1
-    for (int q = 0; q < 4; ++q) {
-      const d4 blk = (q == 0) ? acc[0][0] : ((q == 1) ? acc[0][1] : ((q == 2) ? acc[1][0] : acc[1][1]));
+      for (int q = 0; q < 4; ++q) {
+        const d4 blk = (q == 0) ? acc[0][0] : ((q == 1) ? acc[0][1] : ((q == 2) ? acc[1][0] : acc[1][1]));
 #pragma unroll
-      for (int e = 0; e < 4; ++e) element(acc_row(lane, wr, q >> 1), acc_col(lane, wc, q & 1, e), blk[e]);
+        for (int e = 0; e < 4; ++e) element(ktag, acc_row(lane, wr, q >> 1), acc_col(lane, wc, q & 1, e), blk[e]);
+      }
+    };
+    switch (kern) {
+      case K_GAUSS: blocks(std::integral_constant<int, K_GAUSS>{}); break;
+      case K_EXP: blocks(std::integral_constant<int, K_EXP>{}); break;
+      case K_MATERN32: blocks(std::integral_constant<int, K_MATERN32>{}); break;
+      default: blocks(std::integral_constant<int, K_MATERN52>{}); break;
     }
   }
 
