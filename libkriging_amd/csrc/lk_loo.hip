@@ -21,6 +21,8 @@
 // B w = L^-T (L^-1 w) - G' (G w) takes two triangular matrix-vector passes.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <math.h>
 
 #include "lk_device.hpp"
@@ -330,34 +332,44 @@ __global__ __launch_bounds__(TPB) void k_loo_adjoint(const Slot* __restrict__ sl
 #pragma unroll
   for (int k = 0; k < DT; ++k) s[k] = 0.0;
   // one element (r, c) of the tile, Bs Bs' value v
-  auto element = [&](int r, int c, double v) {
+  // specialised per covariance kernel: one kernel's code runs
+  auto element = [&](auto ktag, int r, int c, double v) __attribute__((always_inline)) {
+    constexpr int KT = decltype(ktag)::value;
     const int gi = i0 + r, gj = j0 + c;
     if (gi >= n || gj >= gi) return;
     const double m = 2.0 * v - sbi[r] * sxj[c] - sbj[c] * sxi[r];
     double dx[DT], g[DT];
 #pragma unroll
     for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? (sXj[k * NB + c] - sXi[k * NB + r]) : 0.0;
-    const double mr = m * factor * kern_rho_g<DT>(S.kernel, dx, it_, it2, g);
+    const double mr = m * factor * kern_rho_g<DT>(KT, dx, it_, it2, g);
 #pragma unroll
     for (int k = 0; k < DT; ++k) s[k] = fma(mr, g[k], s[k]);
   };
-  if constexpr (DT <= 8) {
+  auto tile = [&](auto ktag) __attribute__((always_inline)) {
+    if constexpr (DT <= 8) {
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+      for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-      for (int bj = 0; bj < 2; ++bj)
+        for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) element(acc_row(lane, wr, bi), acc_col(lane, wc, bj, e), acc[bi][bj][e]);
-  } else {
-    // beyond 8 dimensions the sixteen elements do not unroll: the four
-    // 16 x 16 blocks in a loop, each picked from the accumulators by
-    // selects (a loop index into acc would move it to scratch memory)
+          for (int e = 0; e < 4; ++e) element(ktag, acc_row(lane, wr, bi), acc_col(lane, wc, bj, e), acc[bi][bj][e]);
+    } else {
+      // beyond 8 dimensions the sixteen elements do not unroll: the four
+      // 16 x 16 blocks in a loop, each picked from the accumulators by
+      // selects (a loop index into acc would move it to scratch memory)
 #pragma unroll 1
-    for (int q = 0; q < 4; ++q) {
-      const d4 blk = (q == 0) ? acc[0][0] : ((q == 1) ? acc[0][1] : ((q == 2) ? acc[1][0] : acc[1][1]));
+      for (int q = 0; q < 4; ++q) {
+        const d4 blk = (q == 0) ? acc[0][0] : ((q == 1) ? acc[0][1] : ((q == 2) ? acc[1][0] : acc[1][1]));
 #pragma unroll
-      for (int e = 0; e < 4; ++e) element(acc_row(lane, wr, q >> 1), acc_col(lane, wc, q & 1, e), blk[e]);
+        for (int e = 0; e < 4; ++e) element(ktag, acc_row(lane, wr, q >> 1), acc_col(lane, wc, q & 1, e), blk[e]);
+      }
     }
+  };
+  switch (S.kernel) {
+    case K_GAUSS: tile(std::integral_constant<int, K_GAUSS>{}); break;
+    case K_EXP: tile(std::integral_constant<int, K_EXP>{}); break;
+    case K_MATERN32: tile(std::integral_constant<int, K_MATERN32>{}); break;
+    default: tile(std::integral_constant<int, K_MATERN52>{}); break;
   }
   __shared__ double sred[4][DT];
 #pragma unroll
