@@ -982,7 +982,8 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
 // recomputed from X (KrigingImpl.cpp:871-884, Kriging.cpp:303-316):
 //   s1_k += (x_i x_j - sigma2 Rinv_ij) R_ij g_k   (the finaliser's
 //           s1 / sigma2 - s2 in one sum; the s2 partials are zero)
-//   a1   += x_i x_j rho_ij        a2   += Rinv_ij rho_ij
+//   a1   += x_i x_j rho_ij        a2   += Rinv_ij rho_ij   (nugget models
+//           whose parameter vector has alpha only)
 // and diagonal elements add x_i^2 and Rinv_ii.  Partials per tile go to
 // red[grad_off + tile * GRAD_STRIDE]: [s1 (DT) | s2 (DT) | a1 a2 dg1 dg2].
 // LMP: the log-marginal-posterior gradient (KrigingImpl.cpp:887-921) has the
@@ -992,7 +993,7 @@ __global__ void k_finalize_ll(const Slot* __restrict__ slots, const SlotParams* 
 // grid (nblk_max (nblk_max + 1) / 2, nslots), 256 threads.  Waves per SIMD:
 // 4 up to 8 dimensions (128 registers: 0.717 -> 0.680 ms per 64 x n = 1024
 // launch against 3 waves; the diagonal elements' sums are taken before the
-// element loop, which keeps the last few values from spilling), 2
+// element loop; the 8-dimension instance spills a few values), 2
 // for 9..16 dimensions (1 before: 9.5 -> 8 ms at n = 8192, d = 10).  Beyond
 // 8 dimensions the epilogue runs its four 16 x 16 blocks as a loop (the
 // sixteen elements are not unrolled there): with the block picked by
