@@ -848,37 +848,43 @@ __global__ __launch_bounds__(MV_TPB) void k_solve_x(const Slot* __restrict__ slo
   // given as input had no panel step: then all of L is read)
   const int lend = (params[b].mode == MODE_FACTOR) ? ld : (ts.tile + 1) * NB;
   const double* lpart = S.red + red_lcol_off(S.nblk);
-  constexpr int XU = 4;  // 2 XU independent loads per lane per batch
-  for (int cc = w; cc < NB; cc += MV_W) {
-    const int c = ts.tile * NB + cc;
-    const double* __restrict__ Wc = S.W + (size_t)c * ld;
-    const double* __restrict__ Ac = S.A + (size_t)c * ld;
-    double s = 0.0, aw = 0.0, al = 0.0;
-    for (int b0 = c; b0 < ld; b0 += 64 * XU) {
-      const int ib = b0 + lane;
-      double wv[XU], ev[XU];
+  // wave w takes the block's columns c0 .. c0 + 7 together: one pass down
+  // the rows with one load of E* and eight coalesced column loads in flight
+  // per row group (the reductions run once per column at the end)
+  constexpr int CW = NB / MV_W;  // columns per wave
+  const int c0 = ts.tile * NB + CW * w;
+  double s[CW], aw[CW];
 #pragma unroll
-      for (int u = 0; u < XU; ++u) {
-        const int i = min(ib + 64 * u, ld - 1);
-        wv[u] = gload(Wc + i);
-        ev[u] = gload(e + i);
+  for (int q = 0; q < CW; ++q) {
+    s[q] = 0.0;
+    aw[q] = 0.0;
+  }
+  for (int i = c0 + lane; i < ld; i += 64) {
+    const double ev = gload(e + i);
+    double wv[CW];
+#pragma unroll
+    for (int q = 0; q < CW; ++q) wv[q] = gload(S.W + (size_t)(c0 + q) * ld + i);
+#pragma unroll
+    for (int q = 0; q < CW; ++q)
+      if (i >= c0 + q) {  // the lower triangle only
+        s[q] = fma(wv[q], ev, s[q]);
+        aw[q] += fabs(wv[q]);
       }
+  }
 #pragma unroll
-      for (int u = 0; u < XU; ++u)
-        if (ib + 64 * u < ld) {
-          s = fma(wv[u], ev[u], s);
-          aw += fabs(wv[u]);
-        }
-    }
+  for (int q = 0; q < CW; ++q) {
+    const int c = c0 + q;
+    const double* __restrict__ Ac = S.A + (size_t)c * ld;
+    double al = 0.0;
     for (int i = c + lane; i < lend; i += 64) al += fabs(gload(Ac + i));
     if (lend < ld)
       for (int r = ts.tile + 1 + lane; r < S.nblk; r += 64) al += lpart[(size_t)r * ld + c];
-    s = wave_sum(s);
-    aw = wave_sum(aw);
+    const double sx = wave_sum(s[q]);
+    const double awx = wave_sum(aw[q]);
     al = wave_sum(al);
-    if (lane == 0) x[c] = s;
+    if (lane == 0) x[c] = sx;
     if (c < n) {
-      mW = fmax(mW, aw);
+      mW = fmax(mW, awx);
       mL = fmax(mL, al);
     }
   }
