@@ -124,7 +124,7 @@ __global__ __launch_bounds__(TPB) void k_build(const Slot* __restrict__ slots, c
       } else {
 #pragma unroll
         for (int k = 0; k < DT; ++k) dx[k] = (k < d) ? sXj[k * NB + jj] - sXi[k * NB + ii] : 0.0;
-        v = kern_rho<DT>(KT, dx, it, DT <= 16 ? d : DT) * factor;  // padded coordinates skipped
+        v = kern_rho<DT>(KT, dx, it, DT <= 8 ? d : DT) * factor;  // padded coordinates skipped up to 8 dimensions (beyond, the branches cost more than they save: n = 8192, d = 10 build 0.216 vs 0.143 ms)
       }
       S.A[(size_t)gj * S.npad + gi] = v;
     }
