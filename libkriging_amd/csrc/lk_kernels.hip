@@ -1257,6 +1257,51 @@ __global__ __launch_bounds__(TPB, (DT == 16 || DT == 12 ? 2 : (DT <= 8 ? 4 : 1))
 // (7) gradient (Kriging.cpp:303-326): sums of the tile partials in tile order
 // ---------------------------------------------------------------------------
 constexpr int FG_TPB = 1024;  // 16 waves
+// a slot of at least FG_SPLIT_MIN tiles (n >= 4096) is summed in two levels:
+// FG_GROUPS workgroups each add a contiguous run of its tiles in tile order
+// and leave the sum in the run's first tile row, then k_finalize_grad adds
+// those FG_GROUPS rows in order (one workgroup alone reads the 4.5 MB of an
+// n = 8192 slot's partials at a fraction of the chip's bandwidth)
+constexpr int FG_SPLIT_MIN = 2048;
+constexpr int FG_GROUPS = 128;
+constexpr int FG1_TPB = 256;
+__device__ inline int fg_group_start(int ntiles, int g) { return (int)(((long long)ntiles * g) / FG_GROUPS); }
+
+__global__ __launch_bounds__(FG1_TPB) void k_grad_groups(const Slot* __restrict__ slots,
+                                                         const SlotParams* __restrict__ params,
+                                                         SlotOut* __restrict__ out) {
+  const int g = blockIdx.x, b = blockIdx.y;
+  const Slot S = slots[b];
+  const SlotParams& P = params[b];
+  if (!slot_live(P, out[b]) || !P.want_grad) return;
+  const int ntiles = S.nblk * (S.nblk + 1) / 2;
+  if (ntiles < FG_SPLIT_MIN) return;
+  constexpr int NW = FG1_TPB / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double* src = S.red + red_grad_off(S.nblk);
+  const int g0 = fg_group_start(ntiles, g), g1 = fg_group_start(ntiles, g + 1);
+  const int t0 = g0 + (g1 - g0) * w / NW, t1 = g0 + (g1 - g0) * (w + 1) / NW;
+  __shared__ double part[NW][GRAD_STRIDE];
+  for (int q = lane; q < GRAD_STRIDE; q += 64) {
+    double sm[4] = {0.0, 0.0, 0.0, 0.0};
+    int t = t0;
+    for (; t + 3 < t1; t += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) sm[u] += src[(size_t)(t + u) * GRAD_STRIDE + q];
+    }
+    for (; t < t1; ++t) sm[0] += src[(size_t)t * GRAD_STRIDE + q];
+    part[w][q] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
+  }
+  // every read of this run is done before its first row is overwritten;
+  // the runs of the other workgroups are disjoint
+  __syncthreads();
+  for (int q = threadIdx.x; q < GRAD_STRIDE; q += FG1_TPB) {
+    double s = part[0][q];
+#pragma unroll
+    for (int v = 1; v < NW; ++v) s += part[v][q];
+    src[(size_t)g0 * GRAD_STRIDE + q] = s;
+  }
+}
 
 __global__ __launch_bounds__(FG_TPB) void k_finalize_grad(const Slot* __restrict__ slots,
                                                           const SlotParams* __restrict__ params,
@@ -1276,7 +1321,18 @@ __global__ __launch_bounds__(FG_TPB) void k_finalize_grad(const Slot* __restrict
   // partials are contiguous: coalesced rows), eight running sums in tile
   // order (eight loads in flight); the sixteenths are added in order:
   // deterministic for a given tile count
-  {
+  if (ntiles >= FG_SPLIT_MIN) {
+    // the FG_GROUPS run sums of k_grad_groups, in run order
+    const double* src = S.red + red_grad_off(S.nblk);
+    constexpr int RW = FG_GROUPS / NW;  // runs per wave
+    static_assert(FG_GROUPS % NW == 0 && FG_SPLIT_MIN >= 16 * FG_GROUPS, "whole runs per wave, >= 16 tiles a run");
+    for (int q = lane; q < GRAD_STRIDE; q += 64) {
+      double sm = 0.0;
+#pragma unroll
+      for (int u = 0; u < RW; ++u) sm += src[(size_t)fg_group_start(ntiles, w * RW + u) * GRAD_STRIDE + q];
+      part[w][q] = sm;
+    }
+  } else {
     const double* src = S.red + red_grad_off(S.nblk);
     const int t0 = (int)(((long long)ntiles * w) / NW);
     const int t1 = (int)(((long long)ntiles * (w + 1)) / NW);
@@ -2039,6 +2095,12 @@ static hipError_t launch_lauum_adjoint_t(const Batch& B) {
 hipError_t launch_lauum_adjoint(const Batch& B) { return launch_lauum_adjoint_t<false>(B); }
 hipError_t launch_lmp_adjoint(const Batch& B) { return launch_lauum_adjoint_t<true>(B); }
 hipError_t launch_finalize_grad(const Batch& B) {
+  if (tri(B.nblk_max) >= FG_SPLIT_MIN) {
+    hipLaunchKernelGGL(k_grad_groups, dim3(FG_GROUPS, B.nslots), dim3(FG1_TPB), 0, B.stream, B.slots, B.params,
+                       B.out);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_finalize_grad, dim3(B.nslots), dim3(FG_TPB), 0, B.stream, B.slots, B.params, B.out, B.nslots);
   return hipGetLastError();
 }
