@@ -516,10 +516,12 @@ __device__ inline double kern_rho_g(int kern, const double* dx, const double* it
 }
 
 // The matrix-vector kernels stream the L^-1 (and L) triangles once; they
-// run 8 waves per workgroup and issue MV_U independent loads per lane
-// before the first dependent FMA, so a workgroup keeps ~32 KB in flight
-// instead of one 512-byte line per wave.
-constexpr int MV_TPB = 512;
+// run 16 waves per workgroup and issue MV_U independent loads per lane
+// before the first dependent FMA, so a workgroup keeps ~64 KB in flight
+// instead of one 512-byte line per wave (one workgroup per 64-row block:
+// for one large matrix the last row block's workgroup reads its whole
+// row alone, at what its loads in flight allow).
+constexpr int MV_TPB = 1024;
 constexpr int MV_W = MV_TPB / 64;
 constexpr int MV_U = 8;
 static_assert(4 * MV_U <= 64, "one lane per (input column, batch column)");

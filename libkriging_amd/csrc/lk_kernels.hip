@@ -848,9 +848,9 @@ __global__ __launch_bounds__(MV_TPB) void k_solve_x(const Slot* __restrict__ slo
   // given as input had no panel step: then all of L is read)
   const int lend = (params[b].mode == MODE_FACTOR) ? ld : (ts.tile + 1) * NB;
   const double* lpart = S.red + red_lcol_off(S.nblk);
-  // wave w takes the block's columns c0 .. c0 + 7 together: one pass down
-  // the rows with one load of E* and eight coalesced column loads in flight
-  // per row group (the reductions run once per column at the end)
+  // wave w takes the block's columns c0 .. c0 + CW - 1 together: one pass
+  // down the rows with one load of E* and CW coalesced column loads in
+  // flight per row group (the reductions run once per column at the end)
   constexpr int CW = NB / MV_W;  // columns per wave
   const int c0 = ts.tile * NB + CW * w;
   double s[CW], aw[CW];
