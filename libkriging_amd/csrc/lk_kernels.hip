@@ -699,7 +699,12 @@ __global__ __launch_bounds__(TPB, 4) void k_trtri(const Slot* __restrict__ slots
   const int per_pair = s * s;
   const int pr = ts.tile / per_pair;
   const int rem = ts.tile % per_pair;
-  const int ti = rem / s, tj = rem % s;
+  // tiles in order of falling K length (longest first, the short ones fill
+  // the tail), and consecutive workgroups, which the dispatcher deals to the
+  // 8 XCDs in turn, with equal K: step 0's K falls with tj, step 1's grows
+  // with ti
+  const int ti = (step == 0) ? rem % s : s - 1 - rem / s;
+  const int tj = (step == 0) ? rem / s : rem % s;
   const int top = 2 * pr * s, bot = top + s;
   const int it = bot + ti, jt = top + tj;  // tile row / col of the output
   if (bot >= S.nblk || it >= S.nblk) return;
