@@ -36,7 +36,22 @@ Runtime::Runtime(int device) : device_(device) {
   hip_check(hipSetDevice(device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
   for (auto& e : la_ev_) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-  if (const char* e = std::getenv("LKGPU_LA_RESERVE")) la_reserve_ = std::max(0, std::atoi(e));
+  {
+    int cus = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "CU count");
+    if (const char* e = std::getenv("LKGPU_LA_RESERVE")) la_reserve_ = std::max(0, std::atoi(e));
+    if (la_reserve_ > 0 && la_reserve_ < cus) {
+      // the bulk update's stream on the first cus - la_reserve_ CUs
+      // (n = 8192 Cholesky: 6.44 / 6.44 / 6.60 ms with 24 / 32 / 48 CUs kept
+      // free, 7.00 ms without a mask, 7.05 ms without the look-ahead)
+      std::vector<uint32_t> mask((cus + 31) / 32, 0u);
+      for (int c = 0; c < cus - la_reserve_; ++c) mask[c / 32] |= 1u << (c % 32);
+      hip_check(hipExtStreamCreateWithCUMask(&la_stream_, (uint32_t)mask.size(), mask.data()),
+                "hipExtStreamCreateWithCUMask");
+    } else {
+      hip_check(hipStreamCreateWithFlags(&la_stream_, hipStreamNonBlocking), "hipStreamCreate");
+    }
+  }
   if (const char* e = std::getenv("LKGPU_LOOKAHEAD")) lookahead_ = std::atoi(e) != 0;
   if (const char* e = std::getenv("LKGPU_LOOKAHEAD_WG")) la_workgroups_ = std::max(0, std::atoi(e));
   // the allocation-free entry point's scratch: its device address, looked
@@ -772,36 +787,12 @@ void Runtime::issue_split(const std::vector<Step>* parts, int S) {
   }
 }
 
-// The look-ahead's second stream, created on the first look-ahead
-// evaluation rather than with the context: while a CU-masked stream exists,
-// a batch split over the main stream and one more ran 1.5x slower (3.68 vs
-// 2.49 ms per 64 x n = 1024 batch; unmasked, or with three extra streams,
-// it did not), so contexts that never factor one large matrix hold none.
-void Runtime::ensure_lookahead_stream() {
-  if (la_stream_ != nullptr) return;
-  hip_check(hipSetDevice(device_), "hipSetDevice");
-  int cus = 0;
-  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "CU count");
-  if (la_reserve_ > 0 && la_reserve_ < cus) {
-    // the bulk update's stream on the first cus - la_reserve_ CUs
-    // (n = 8192 Cholesky: 6.44 / 6.44 / 6.60 ms with 24 / 32 / 48 CUs kept
-    // free, 7.00 ms without a mask, 7.05 ms without the look-ahead)
-    std::vector<uint32_t> mask((cus + 31) / 32, 0u);
-    for (int c = 0; c < cus - la_reserve_; ++c) mask[c / 32] |= 1u << (c % 32);
-    hip_check(hipExtStreamCreateWithCUMask(&la_stream_, (uint32_t)mask.size(), mask.data()),
-              "hipExtStreamCreateWithCUMask");
-  } else {
-    hip_check(hipStreamCreateWithFlags(&la_stream_, hipStreamNonBlocking), "hipStreamCreate");
-  }
-}
-
 void Runtime::run_pipeline(bool want_grad, BenchMap* bench, int objective) {
   hip_check(hipMemcpyAsync(d_params_, h_params_, sizeof(SlotParams) * nslots_, hipMemcpyHostToDevice, stream_),
             "upload params");
   Part all{0, nslots_, nblk_max_, dmax_};
   apply_tuning(all);
   const bool la = lookahead_active();
-  if (la) ensure_lookahead_stream();
   const std::vector<Step> steps = pipeline_steps(part_batch(all, stream_), want_grad, objective, all.panel_blocks,
                                                  all.wide, la);
   const auto rng = stage_range(steps, timed_stage_);

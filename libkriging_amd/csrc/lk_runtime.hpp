@@ -194,7 +194,6 @@ class Runtime {
   // results are the same either way
   void set_lookahead(bool on) { lookahead_ = on; ++layout_version_; }
   bool lookahead_active() const { return lookahead_ && nslots_ == 1 && nblk_max_ >= kLookaheadMinBlocks; }
-  void ensure_lookahead_stream();
   // parts of the last run (slot ranges in bucket order) and their bucket
   int last_parts() const { return last_parts_; }
 
